@@ -1,0 +1,222 @@
+/*
+ * hcs.h -- C-ABI of the MI355X hydroelastic contact-surface engine (libhcs).
+ *
+ * Plain C, no exceptions cross it, every call returns an int status
+ * (HCS_OK = 0, negative on error; hcs_last_error() gives the text).  The
+ * caller owns host buffers, the context owns device memory.  A context is
+ * used from one host thread at a time (the reference keys one plugin
+ * instance per mjData, plugin.cpp:88,225).
+ *
+ * Each entry point names the reference interface it replaces; citations are
+ * relative to the reference repository (P/ = mujoco_contact_surfaces/,
+ * S/ = mujoco_contact_surface_sensors/, plugin.cpp =
+ * P/src/mujoco_contact_surfaces_plugin.cpp).
+ *
+ * Conventions shared by every entry point:
+ *   pose X (12 doubles)  = geom_xmat (row-major 3x3) followed by geom_xpos,
+ *                          as read by getGeomPose (plugin.cpp:116-126);
+ *   twist V (6 doubles)  = [w; v] of the geom origin in world axes, as
+ *                          returned by mj_objectVelocity(..., flg_local = 0)
+ *                          in getGeomVelocity (plugin.cpp:107-114);
+ *   all arithmetic is FP64.
+ */
+#ifndef HCS_H
+#define HCS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HCS_OK 0
+#define HCS_ERR_ARG (-1)      /* invalid argument, handle or size */
+#define HCS_ERR_HIP (-2)      /* HIP runtime error (text in hcs_last_error) */
+#define HCS_ERR_OOM (-3)      /* device allocation failed */
+#define HCS_ERR_CAPACITY (-4) /* an output buffer was too small; counts are still valid */
+#define HCS_ERR_STATE (-5)    /* call order violated (e.g. taxels before a contact step) */
+#define HCS_ERR_LIMIT (-6)    /* an internal limit was exceeded (e.g. BVH deeper than the traversal stack) */
+
+typedef struct hcs_ctx hcs_ctx;
+typedef struct hcs_batch hcs_batch;
+
+/* The cs::<geom> 5-tuple read by parseMujocoCustomFields
+ * (plugin.cpp:613-626).  hydroelastic_modulus > 0 is a soft geometry; 0 (or
+ * +inf) is rigid, whose modulus is +inf and dissipation 1
+ * (P/include/mujoco_contact_surfaces/mujoco_contact_surfaces_plugin.h:192-213). */
+typedef struct hcs_props {
+  double hydroelastic_modulus;
+  double dissipation;
+  double resolution_hint;
+  double static_friction;
+  double dynamic_friction;
+} hcs_props;
+
+/* One geometry pair of one step: what collision_cb receives for (g1, g2)
+ * (plugin.cpp:255) plus the poses and twists passiveCallback reads
+ * (plugin.cpp:435-438). */
+typedef struct hcs_pair_in {
+  int32_t geom_a, geom_b; /* mesh handles from hcs_add_*_mesh */
+  int32_t reserved[2];
+  double X_WA[12], X_WB[12];
+  double V_WA[6], V_WB[6];
+} hcs_pair_in;
+
+/* Per-pair result: the net effect of the per-point mj_applyFT calls of
+ * passiveCallback (plugin.cpp:477-482) expressed as one wrench per geometry
+ * about its own origin (mj_applyFT is linear in force, torque and point). */
+typedef struct hcs_pair_out {
+  double F_A[3], tau_A[3];
+  double F_B[3], tau_B[3];
+  int64_t n_candidates; /* |C|: element pairs with overlapping AABBs */
+  int64_t n_polygons;   /* non-empty contact polygons */
+  int64_t n_faces;      /* PointCollision records (faces passing the filters of plugin.cpp:345,361-368) */
+  int64_t status;       /* 0, or an HCS_ERR_* code for this pair */
+} hcs_pair_out;
+
+/* Step options: hydroelastic_contact_representation (plugin.h:251; 0 =
+ * kTriangle, the reference default, 1 = kPolygon), cs::ApplyContactSurfaceForces
+ * (plugin.cpp:599-611), the time constant hard-coded as 0.001 at
+ * plugin.cpp:458, and whether to keep the contact polygons on the device for
+ * hcs_taxels. */
+typedef struct hcs_step_opts {
+  int32_t representation;
+  int32_t apply_forces;
+  double dt;
+  int32_t keep_polygons;
+  int32_t reserved;
+} hcs_step_opts;
+
+/* Optional host copy of the PointCollision list built by
+ * evaluateContactSurface (plugin.cpp:404; struct at
+ * P/include/mujoco_contact_surfaces/common_types.h:48-56).
+ * rec: capacity x 12 doubles = p[3], n[3], fn0, stiffness, damping, area, p0, g
+ * key: capacity x 4 int64   = pair, element of A, element of B, fan index
+ * Records come in no particular order; sort by key for a canonical order. */
+typedef struct hcs_faces {
+  int64_t capacity;
+  int64_t count; /* out: faces produced (may exceed capacity -> HCS_ERR_CAPACITY) */
+  double* rec;
+  int64_t* key;
+} hcs_faces;
+
+/* Flat tactile sensor of FlatTactileSensor::load/bvh_update
+ * (S/src/flat_tactile_sensor.cpp:127-214, 262-402).  The sensor samples the
+ * contact polygons of every pair of the last contact step that involves mesh
+ * `geom` (the reference's gc->g1 == id or gc->g2 == id test). */
+typedef struct hcs_taxel_grid {
+  int32_t geom;                /* mesh handle of the sensor geometry */
+  int32_t sampling_resolution; /* S: S x S rays per taxel */
+  int32_t window;              /* 0 none, 1 gauss, 2 tukey, 3 square */
+  int32_t reserved;
+  double sigma;
+  double X_W_sensor[12];       /* sensor geom pose */
+  double half_size[3];         /* geom_size: xs, ys, zs */
+  double resolution;           /* taxel pitch */
+} hcs_taxel_grid;
+
+/* ---- lifetime (plugin constructor/load/destructor, plugin.cpp:191-245) */
+int hcs_create(hcs_ctx** ctx, int device_ordinal);
+int hcs_destroy(hcs_ctx* ctx);
+const char* hcs_last_error(const hcs_ctx* ctx); /* NULL ctx: last create error */
+int hcs_synchronize(hcs_ctx* ctx);
+/* The hipStream_t every launch of this context uses (default priority). */
+void* hcs_stream(hcs_ctx* ctx);
+
+/* ---- geometry (soft and rigid branches of parseMujocoCustomFields,
+ * plugin.cpp:613-808).  Soft: vertices nv x 3 in the geom frame, positively
+ * oriented tets nt x 4, per-vertex pressure (the linear field of
+ * VolumeMeshFieldLinear).  Builds the tet records and the static LBVH on the
+ * device.  Rigid: vertices nv x 3 and triangles ntri x 3 (outward
+ * counter-clockwise).  geom_id orders M/N like Drake's GeometryId
+ * (plugin.cpp:307-311): the geometry with the smaller id is M. */
+int hcs_add_soft_mesh(hcs_ctx* ctx, const double* V, int32_t nv, const int32_t* T, int32_t nt,
+                      const double* pressure, const hcs_props* props, int32_t geom_id, int32_t* handle);
+int hcs_add_rigid_mesh(hcs_ctx* ctx, const double* V, int32_t nv, const int32_t* F, int32_t ntri,
+                       const hcs_props* props, int32_t geom_id, int32_t* handle);
+/* info = [kind (1 soft, 2 rigid), nv, elements, bvh height] */
+int hcs_mesh_info(hcs_ctx* ctx, int32_t handle, int64_t* info);
+/* Device tet records of a soft mesh, nt x 24 doubles (unit outward face
+ * normals [12], plane offsets [4], pressure gradient [3], constant [1],
+ * normalised gradient [3], unused [1]); for parity checks. */
+int hcs_mesh_tet_records(hcs_ctx* ctx, int32_t handle, double* out);
+
+/* ---- per-step contact (collision_cb + Drake query + evaluateContactSurface
+ * + the passiveCallback force loop; plugin.cpp:255-483).  Soft-rigid and
+ * soft-soft pairs; rigid-rigid pairs are rejected (the reference falls back
+ * to MuJoCo there, plugin.cpp:271).  faces may be NULL. */
+int hcs_contact_step(hcs_ctx* ctx, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts,
+                     hcs_pair_out* out, hcs_faces* faces);
+
+/* Canonical candidate keys of one pair for parity checks:
+ * key = elemA * n_elements(B) + elemB, sorted ascending. */
+int hcs_candidates(hcs_ctx* ctx, const hcs_pair_in* pair, int64_t* keys, int64_t capacity, int64_t* count);
+
+/* ---- tactile sensors (FlatTactileSensor::bvh_update), on the polygons of
+ * the last hcs_contact_step run with keep_polygons = 1.  out holds, per
+ * sensor, cx * cy values at [x + cx * y] with cx = floor(2 xs / res + 0.1)
+ * (S/src/flat_tactile_sensor.cpp:196-197). */
+int hcs_taxel_dims(const hcs_taxel_grid* g, int32_t* cx, int32_t* cy);
+int hcs_taxels(hcs_ctx* ctx, int32_t nsensors, const hcs_taxel_grid* g, double* out);
+
+/* ---- batched form (no reference counterpart: the reference runs one pair
+ * at a time inside mj_collision).  Thousands of independent pairs, one
+ * persistent work-queue launch per step, captured in a hipGraph. */
+int hcs_batch_create(hcs_ctx* ctx, int32_t npairs, const int32_t* geoms /* npairs x 2 */,
+                     const hcs_step_opts* opts, hcs_batch** batch);
+int hcs_batch_destroy(hcs_batch* batch);
+/* state: npairs x 36 doubles = X_WA, X_WB, V_WA, V_WB (host, copied to the device) */
+int hcs_batch_set_state(hcs_batch* batch, const double* state);
+/* One contact step for every pair (graph replay; asynchronous on hcs_stream). */
+int hcs_batch_step(hcs_batch* batch);
+int hcs_batch_get(hcs_batch* batch, hcs_pair_out* out);
+/* Autotuner: times the launch shapes on the current state, keeps the best. */
+int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
+/* Launch configuration: [block waves, persistent blocks per CU, items of tri-mode chunk] */
+int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
+/* Eager per-kernel timing with HIP events on hcs_stream: ms[0] prep,
+ * ms[1] query/clip, ms[2] reduce; averages over reps. */
+int hcs_batch_kernel_times(hcs_batch* batch, int32_t reps, double* ms);
+/* Work-unit counts of the last step: [candidates, polygons, faces]. */
+int hcs_batch_counts(hcs_batch* batch, int64_t* counts);
+
+/* ---- differentiation (no reference counterpart).  Tangent-linear
+ * recomputation of one step at fixed clipping topology:
+ * J (npairs x 12 x 7): d(F_A, tau_A, F_B, tau_B) / d(xi), xi = [dtheta (3),
+ * dt (3) of the pose of geometry B, applied as X_WB <- (exp([dtheta]) R_WB,
+ * t_WB + dt), and dE of geometry A's modulus (pressures scale with E)].
+ * Soft-rigid pairs take A soft and B rigid. */
+int hcs_contact_jacobian(hcs_ctx* ctx, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts,
+                         double* J);
+/* Vector-Jacobian product: dL_dxi (npairs x 7) = J^T dL_dout (npairs x 12). */
+int hcs_contact_backward(hcs_ctx* ctx, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts,
+                         const double* dL_dout, double* dL_dxi);
+/* Taxel Jacobian for one sensor attached to geometry B of pair `pair`
+ * (the sensor pose moves with B): Jt (cx*cy x 7). */
+int hcs_taxel_jacobian(hcs_ctx* ctx, const hcs_pair_in* pair, const hcs_step_opts* opts, const hcs_taxel_grid* g,
+                       double* Jt);
+
+/* ---- synthetic geometry (host side; stands in for the Drake mesh and
+ * pressure-field generators called at plugin.cpp:648-796).
+ * Soft bodies: icosphere of level L, K radial layers around one centre
+ * vertex; pressure E (1 - layer/K), E at the centre.
+ * shape 0 sphere (a = radius), 1 ellipsoid (a, b, c semi-axes),
+ * 2 capsule (a = radius, b = half length of the straight part). */
+int hcs_gen_soft_size(int32_t L, int32_t K, int32_t* nv, int32_t* nt);
+int hcs_gen_soft(int32_t L, int32_t K, int32_t shape, double a, double b, double c, double E, double* V,
+                 int32_t* T, double* p);
+/* Rigid surfaces: kind 0 box (a, b, c half sizes, m x m squares per face,
+ * 12 m^2 triangles), 1 plane grid z = 0 of side a with m x m squares
+ * (2 m^2 triangles, fixed diagonal), 2 icosphere of radius a, level m. */
+int hcs_gen_rigid_size(int32_t kind, int32_t m, int32_t* nv, int32_t* ntri);
+int hcs_gen_rigid(int32_t kind, int32_t m, double a, double b, double c, double* V, int32_t* F);
+/* Soft body from a closed surface: one tet per triangle fanned to the
+ * enclosed-volume centroid, pressure 0 on the surface and E at the centroid
+ * (plugin.cpp:743-793 with CalcCentroidOfEnclosedVolume, plugin.cpp:161-187).
+ * V_out: (nv + 1) x 3, T_out: ntri x 4, p_out: nv + 1. */
+int hcs_gen_soft_from_surface(const double* V, int32_t nv, const int32_t* F, int32_t ntri, double E, double* V_out,
+                              int32_t* T_out, double* p_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

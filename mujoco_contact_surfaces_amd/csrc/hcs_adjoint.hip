@@ -1,0 +1,263 @@
+/*
+ * hcs_adjoint.hip -- differentiation of the contact step at fixed clipping
+ * topology (no reference counterpart).
+ *
+ * Tangent-linear recomputation: the query kernel is re-run with the Dual
+ * scalar type, one tangent direction per grid.y slice (rotation and
+ * translation of geometry B's pose, modulus of geometry A), recomputing the
+ * polygons instead of storing them.  Candidate pairs, inside/outside tests,
+ * filters and branches follow the primal values, so the result is the exact
+ * derivative of the piecewise-smooth map on the current piece.  The
+ * vector-Jacobian product (the adjoint) is J^T applied to the output
+ * cotangent; with 7 inputs and 12 (+ taxel) outputs the tangent form is the
+ * cheaper way to get it.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "hcs_device.h"
+#include "hcs_internal.h"
+#include "hcs_query.h"
+
+#pragma clang fp contract(off)
+
+using namespace hcs;
+
+namespace hcs {
+
+static const int kDirs = 7;
+
+/* Uploads plan and state of npairs pairs into the context scratch. */
+int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan) {
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  std::vector<int32_t> geoms(2 * (size_t)npairs);
+  std::vector<double> state((size_t)HCS_STATE * npairs);
+  for (int e = 0; e < npairs; ++e) {
+    geoms[2 * e] = in[e].geom_a, geoms[2 * e + 1] = in[e].geom_b;
+    double* s = &state[(size_t)HCS_STATE * e];
+    std::memcpy(s, in[e].X_WA, 12 * sizeof(double));
+    std::memcpy(s + 12, in[e].X_WB, 12 * sizeof(double));
+    std::memcpy(s + 24, in[e].V_WA, 6 * sizeof(double));
+    std::memcpy(s + 30, in[e].V_WB, 6 * sizeof(double));
+  }
+  rc = build_plan(c, npairs, geoms.data(), opts, 1, c->tet_mode_max_tris, plan);
+  if (rc) return rc;
+  const size_t nitems = std::max<size_t>(1, plan.items.size());
+  if ((rc = ensure(c, c->d_envs, sizeof(EnvDesc) * npairs)) || (rc = ensure(c, c->d_state, state.size() * 8)) ||
+      (rc = ensure(c, c->d_xf, 12 * sizeof(double) * npairs)) || (rc = ensure(c, c->d_items, sizeof(Item) * nitems)) ||
+      (rc = ensure(c, c->d_partials, sizeof(Partial) * nitems * kDirs)) || (rc = ensure(c, c->d_err, 64)) ||
+      (rc = ensure(c, c->d_poly_cnt, 64)))
+    return rc;
+  rc = hip_check(c, hipMemcpyAsync(c->d_envs.ptr, plan.envs.data(), sizeof(EnvDesc) * npairs, hipMemcpyHostToDevice,
+                                   c->stream),
+                 "upload");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(c->d_state.ptr, state.data(), state.size() * 8, hipMemcpyHostToDevice, c->stream),
+                   "upload");
+  if (!rc && !plan.items.empty())
+    rc = hip_check(c, hipMemcpyAsync(c->d_items.ptr, plan.items.data(), sizeof(Item) * plan.items.size(),
+                                     hipMemcpyHostToDevice, c->stream),
+                   "upload");
+  if (!rc) rc = hip_check(c, hipMemsetAsync(c->d_err.ptr, 0, 64, c->stream), "memset");
+  return rc;
+}
+
+/* Per-pair tangent seeds: 1/E_A and d(d*)/dE_A. */
+static void seeds(Ctx* c, const hcs_pair_in* in, int npairs, std::vector<double>& invE, std::vector<double>& ddE) {
+  invE.assign(npairs, 0.0);
+  ddE.assign(npairs, 0.0);
+  for (int e = 0; e < npairs; ++e) {
+    const Mesh& A = c->meshes[in[e].geom_a];
+    const Mesh& B = c->meshes[in[e].geom_b];
+    const double EA = A.props.hydroelastic_modulus;
+    invE[e] = 1.0 / EA;
+    if (B.kind == 1) {
+      /* d* = E_B/(E_A+E_B) d_A + E_A/(E_A+E_B) d_B (plugin.cpp:138-159) */
+      const double EB = B.props.hydroelastic_modulus, s = EA + EB;
+      ddE[e] = EB * (B.props.dissipation - A.props.dissipation) / (s * s);
+    }
+  }
+}
+
+static QueryArgs tangent_args(Ctx* c, const Plan& plan) {
+  QueryArgs a{};
+  a.envs = (const EnvDesc*)c->d_envs.ptr;
+  a.state = (const double*)c->d_state.ptr;
+  a.xf = (const double*)c->d_xf.ptr;
+  a.items = (const Item*)c->d_items.ptr;
+  a.nitems = (int)plan.items.size();
+  a.queries_per_lane = 1;
+  a.counter = nullptr;
+  a.partials = (Partial*)c->d_partials.ptr;
+  a.error = (int*)c->d_err.ptr;
+  return a;
+}
+
+}  // namespace hcs
+
+extern "C" {
+
+int hcs_contact_jacobian(hcs_ctx* h, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts, double* J) {
+  if (!h || npairs < 0 || (npairs > 0 && (!in || !J))) return HCS_ERR_ARG;
+  if (npairs == 0) return HCS_OK;
+  Ctx* c = &h->c;
+  Plan plan;
+  int rc = stage_pairs(c, npairs, in, opts, plan);
+  if (rc) return rc;
+  std::vector<double> invE, ddE;
+  seeds(c, in, npairs, invE, ddE);
+  DBuf dseed;
+  rc = ensure(c, dseed, sizeof(double) * 2 * npairs);
+  if (rc) return rc;
+  std::vector<double> sd(2 * (size_t)npairs);
+  std::copy(ddE.begin(), ddE.end(), sd.begin());
+  std::copy(invE.begin(), invE.end(), sd.begin() + npairs);
+  rc = hip_check(c, hipMemcpyAsync(dseed.ptr, sd.data(), sd.size() * 8, hipMemcpyHostToDevice, c->stream), "upload");
+  const int nitems = (int)plan.items.size();
+  std::vector<Partial> part((size_t)nitems * kDirs);
+  if (!rc && nitems > 0) {
+    QueryArgs a = tangent_args(c, plan);
+    launch_prep(c->stream, a.envs, a.state, (double*)c->d_xf.ptr, npairs);
+    launch_query_tangent(c->stream, a, kDirs, (const double*)dseed.ptr, (const double*)dseed.ptr + npairs,
+                         (Partial*)c->d_partials.ptr);
+    rc = hip_check(c, hipGetLastError(), "tangent query");
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(part.data(), c->d_partials.ptr, sizeof(Partial) * part.size(),
+                                       hipMemcpyDeviceToHost, c->stream),
+                     "download");
+  }
+  int err = 0;
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(&err, c->d_err.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  (void)hipFree(dseed.ptr);
+  if (rc) return rc;
+  if (err) return set_err(c, HCS_ERR_LIMIT, "traversal stack overflow");
+  for (int e = 0; e < npairs; ++e) {
+    const EnvDesc& E = plan.envs[e];
+    double* Je = J + (size_t)e * 12 * kDirs;
+    for (int d = 0; d < kDirs; ++d) {
+      double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
+      for (int i = E.item_first; i < E.item_first + E.item_count; ++i) {
+        const Partial& q = part[(size_t)d * nitems + i];
+        for (int k = 0; k < 3; ++k) F[k] += q.F[k], tA[k] += q.tA[k], tB[k] += q.tB[k];
+      }
+      for (int k = 0; k < 3; ++k) {
+        Je[(0 + k) * kDirs + d] = F[k];
+        Je[(3 + k) * kDirs + d] = tA[k];
+        Je[(6 + k) * kDirs + d] = -F[k];
+        Je[(9 + k) * kDirs + d] = tB[k];
+      }
+    }
+  }
+  return HCS_OK;
+}
+
+int hcs_contact_backward(hcs_ctx* h, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts,
+                         const double* dL_dout, double* dL_dxi) {
+  if (!h || npairs < 0 || (npairs > 0 && (!in || !dL_dout || !dL_dxi))) return HCS_ERR_ARG;
+  std::vector<double> J((size_t)npairs * 12 * kDirs);
+  int rc = hcs_contact_jacobian(h, npairs, in, opts, J.data());
+  if (rc) return rc;
+  for (int e = 0; e < npairs; ++e)
+    for (int d = 0; d < kDirs; ++d) {
+      double s = 0;
+      for (int r = 0; r < 12; ++r) s += J[((size_t)e * 12 + r) * kDirs + d] * dL_dout[12 * (size_t)e + r];
+      dL_dxi[kDirs * (size_t)e + d] = s;
+    }
+  return HCS_OK;
+}
+
+int hcs_taxel_jacobian(hcs_ctx* h, const hcs_pair_in* pair, const hcs_step_opts* opts, const hcs_taxel_grid* G,
+                       double* Jt) {
+  if (!h || !pair || !G || !Jt) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  TaxelGridDev g;
+  if (taxel_grid_dev(*G, g) != HCS_OK) return set_err(c, HCS_ERR_ARG, "invalid taxel grid");
+  const int ncell = g.cx * g.cy;
+  Plan plan;
+  hcs_step_opts o = opts ? *opts : hcs_step_opts{0, 1, 0.001, 0, 0};
+  int rc = stage_pairs(c, 1, pair, &o, plan);
+  if (rc) return rc;
+  std::vector<double> invE, ddE;
+  seeds(c, pair, 1, invE, ddE);
+  const int nitems = (int)plan.items.size();
+  /* polygon capacity from a primal count run */
+  hcs_pair_out po;
+  hcs_step_opts oc = o;
+  oc.keep_polygons = 1;
+  rc = hcs_contact_step(h, 1, pair, &oc, &po, nullptr);
+  if (rc) return rc;
+  const int64_t cap = std::max<int64_t>(1, po.n_polygons);
+  rc = stage_pairs(c, 1, pair, &o, plan);
+  if (rc) return rc;
+  DBuf prec, ptan, pkey, dseed, dXd, dmask, dout;
+  auto cleanup = [&]() {
+    for (DBuf* b : {&prec, &ptan, &pkey, &dseed, &dXd, &dmask, &dout})
+      if (b->ptr) (void)hipFree(b->ptr);
+  };
+  if ((rc = ensure(c, prec, sizeof(double) * HCS_PREC * cap)) || (rc = ensure(c, ptan, sizeof(double) * HCS_PREC * cap)) ||
+      (rc = ensure(c, pkey, sizeof(int64_t) * 3 * cap)) || (rc = ensure(c, dseed, sizeof(double) * 2)) ||
+      (rc = ensure(c, dXd, sizeof(double) * 12)) || (rc = ensure(c, dmask, 8)) ||
+      (rc = ensure(c, dout, sizeof(double) * ncell))) {
+    cleanup();
+    return rc;
+  }
+  const double sd[2] = {ddE[0], invE[0]};
+  const unsigned char one = 1;
+  rc = hip_check(c, hipMemcpyAsync(dseed.ptr, sd, sizeof sd, hipMemcpyHostToDevice, c->stream), "upload");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(dmask.ptr, &one, 1, hipMemcpyHostToDevice, c->stream), "upload");
+  const double* R = pair->X_WB;
+  for (int d = 0; d < kDirs && !rc; ++d) {
+    /* tangent of the sensor pose (the sensor rides on geometry B) */
+    double Xd[12] = {0};
+    if (d < 3) {
+      for (int j = 0; j < 3; ++j) {
+        const double r0 = R[j], r1 = R[3 + j], r2 = R[6 + j];
+        double d0 = 0, d1 = 0, d2 = 0;
+        if (d == 0) d1 = -r2, d2 = r1;
+        if (d == 1) d0 = r2, d2 = -r0;
+        if (d == 2) d0 = -r1, d1 = r0;
+        Xd[j] = d0, Xd[3 + j] = d1, Xd[6 + j] = d2;
+      }
+    } else if (d < 6) {
+      Xd[9 + (d - 3)] = 1.0;
+    }
+    rc = hip_check(c, hipMemcpyAsync(dXd.ptr, Xd, sizeof Xd, hipMemcpyHostToDevice, c->stream), "upload");
+    if (!rc) rc = hip_check(c, hipMemsetAsync(c->d_poly_cnt.ptr, 0, 64, c->stream), "memset");
+    if (rc) break;
+    QueryArgs a = tangent_args(c, plan);
+    a.poly_rec = (double*)prec.ptr;
+    a.poly_key = (int64_t*)pkey.ptr;
+    a.poly_cnt = (unsigned long long*)c->d_poly_cnt.ptr;
+    a.poly_cap = cap;
+    a.poly_tan = (double*)ptan.ptr;
+    launch_prep(c->stream, a.envs, a.state, (double*)c->d_xf.ptr, 1);
+    launch_query_tangent_dir(c->stream, a, d, (const double*)dseed.ptr, (const double*)dseed.ptr + 1,
+                             (Partial*)c->d_partials.ptr);
+    rc = hip_check(c, hipGetLastError(), "tangent query");
+    unsigned long long np = 0;
+    if (!rc) rc = hip_check(c, hipMemcpyAsync(&np, c->d_poly_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream), "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    if (rc) break;
+    const int64_t n = std::min<int64_t>((int64_t)np, cap);
+    rc = run_taxels(c, g, (const unsigned char*)dmask.ptr, (const double*)prec.ptr, (const double*)ptan.ptr,
+                    (const int64_t*)pkey.ptr, n, (const double*)dXd.ptr, (double*)dout.ptr);
+    std::vector<double> col(ncell);
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(col.data(), dout.ptr, sizeof(double) * ncell, hipMemcpyDeviceToHost, c->stream),
+                     "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    for (int k = 0; k < ncell && !rc; ++k) Jt[(size_t)k * kDirs + d] = col[k];
+  }
+  (void)nitems;
+  cleanup();
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,283 @@
+/*
+ * hcs_batch.hip -- batched form: thousands of independent pairs stepped by
+ * one persistent work-queue launch, the step captured once in a hipGraph
+ * and replayed.  No reference counterpart (the reference steps one pair at a
+ * time inside mj_collision, mujoco_contact_surfaces_plugin.cpp:92-95).
+ *
+ * Step = [reset queue counter (fill kernel, no memset node), per-pair pose
+ * composition, persistent query/clip/force kernel, per-pair reduction].
+ * The autotuner times the launch shapes (persistent blocks per CU, queries
+ * per lane, the tet-query threshold for tiny rigid meshes) on the current
+ * state and keeps the fastest; results never depend on the shape.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hcs_internal.h"
+
+using namespace hcs;
+
+struct hcs_batch {
+  Ctx* c = nullptr;
+  int npairs = 0;
+  std::vector<int32_t> geoms;
+  hcs_step_opts opts{};
+  Plan plan;
+  int qpl = 1, blocks_per_cu = 4, tet_max = 64;
+  DBuf envs, state, xf, items, partials, out, err, counter;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int batch_free_graph(hcs_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  return HCS_OK;
+}
+
+static int grid_of(const hcs_batch* b) {
+  const int nitems = (int)b->plan.items.size();
+  return std::max(1, std::min(nitems, b->c->num_cu * b->blocks_per_cu));
+}
+
+static QueryArgs batch_args(hcs_batch* b) {
+  QueryArgs a{};
+  a.envs = (const EnvDesc*)b->envs.ptr;
+  a.state = (const double*)b->state.ptr;
+  a.xf = (const double*)b->xf.ptr;
+  a.items = (const Item*)b->items.ptr;
+  a.nitems = (int)b->plan.items.size();
+  a.queries_per_lane = b->qpl;
+  a.counter = (unsigned int*)b->counter.ptr;
+  a.partials = (Partial*)b->partials.ptr;
+  a.error = (int*)b->err.ptr;
+  return a;
+}
+
+/* Enqueues one step on the context stream. */
+static void batch_enqueue(hcs_batch* b) {
+  hipStream_t s = b->c->stream;
+  QueryArgs a = batch_args(b);
+  launch_fill_u32(s, a.counter, 0u, 1);
+  launch_prep(s, a.envs, a.state, (double*)b->xf.ptr, b->npairs);
+  launch_query(s, a, grid_of(b));
+  launch_reduce(s, a.envs, a.partials, a.error, (PairOut*)b->out.ptr, b->npairs);
+}
+
+/* (Re)builds the plan for the current shape and uploads it. */
+static int batch_plan(hcs_batch* b) {
+  Ctx* c = b->c;
+  batch_free_graph(b);
+  int rc = build_plan(c, b->npairs, b->geoms.data(), &b->opts, b->qpl, b->tet_max, b->plan);
+  if (rc) return rc;
+  const size_t nitems = std::max<size_t>(1, b->plan.items.size());
+  if ((rc = ensure(c, b->envs, sizeof(EnvDesc) * b->npairs)) ||
+      (rc = ensure(c, b->state, sizeof(double) * HCS_STATE * b->npairs)) ||
+      (rc = ensure(c, b->xf, sizeof(double) * 12 * b->npairs)) || (rc = ensure(c, b->items, sizeof(Item) * nitems)) ||
+      (rc = ensure(c, b->partials, sizeof(Partial) * nitems)) || (rc = ensure(c, b->out, sizeof(PairOut) * b->npairs)) ||
+      (rc = ensure(c, b->err, 64)) || (rc = ensure(c, b->counter, 64)))
+    return rc;
+  rc = hip_check(c, hipMemcpyAsync(b->envs.ptr, b->plan.envs.data(), sizeof(EnvDesc) * b->npairs,
+                                   hipMemcpyHostToDevice, c->stream),
+                 "upload envs");
+  if (rc) return rc;
+  if (!b->plan.items.empty())
+    rc = hip_check(c, hipMemcpyAsync(b->items.ptr, b->plan.items.data(), sizeof(Item) * b->plan.items.size(),
+                                     hipMemcpyHostToDevice, c->stream),
+                   "upload items");
+  if (rc) return rc;
+  rc = hip_check(c, hipMemsetAsync(b->err.ptr, 0, 64, c->stream), "memset");
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+static int batch_capture(hcs_batch* b) {
+  Ctx* c = b->c;
+  batch_free_graph(b);
+  int rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+  if (rc) return rc;
+  batch_enqueue(b);
+  rc = hip_check(c, hipStreamEndCapture(c->stream, &b->graph), "EndCapture");
+  if (rc) return rc;
+  return hip_check(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "GraphInstantiate");
+}
+
+extern "C" {
+
+int hcs_batch_create(hcs_ctx* h, int32_t npairs, const int32_t* geoms, const hcs_step_opts* opts,
+                     hcs_batch** out) {
+  if (!h || !geoms || !out || npairs <= 0) return HCS_ERR_ARG;
+  *out = nullptr;
+  Ctx* c = &h->c;
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  hcs_batch* b = new hcs_batch();
+  b->c = c;
+  b->npairs = npairs;
+  b->geoms.assign(geoms, geoms + 2 * (size_t)npairs);
+  if (opts) {
+    b->opts = *opts;
+  } else {
+    b->opts.representation = 0, b->opts.apply_forces = 1, b->opts.dt = 0.001;
+  }
+  b->opts.keep_polygons = 0;
+  rc = batch_plan(b);
+  if (!rc) {
+    std::vector<double> zero((size_t)HCS_STATE * npairs, 0.0);
+    for (int e = 0; e < npairs; ++e) {
+      double* s = &zero[(size_t)HCS_STATE * e];
+      s[0] = s[4] = s[8] = s[12] = s[16] = s[20] = 1.0;
+    }
+    rc = hcs_batch_set_state(b, zero.data());
+  }
+  if (!rc) rc = batch_capture(b);
+  if (rc) {
+    hcs_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return HCS_OK;
+}
+
+int hcs_batch_destroy(hcs_batch* b) {
+  if (!b) return HCS_ERR_ARG;
+  (void)hipStreamSynchronize(b->c->stream);
+  batch_free_graph(b);
+  for (DBuf* d : {&b->envs, &b->state, &b->xf, &b->items, &b->partials, &b->out, &b->err, &b->counter})
+    if (d->ptr) (void)hipFree(d->ptr);
+  delete b;
+  return HCS_OK;
+}
+
+int hcs_batch_set_state(hcs_batch* b, const double* state) {
+  if (!b || !state) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  int rc = hip_check(c, hipMemcpyAsync(b->state.ptr, state, sizeof(double) * HCS_STATE * b->npairs,
+                                       hipMemcpyHostToDevice, c->stream),
+                     "upload state");
+  if (rc) return rc;
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int hcs_batch_step(hcs_batch* b) {
+  if (!b) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  if (b->exec) return hip_check(c, hipGraphLaunch(b->exec, c->stream), "hipGraphLaunch");
+  batch_enqueue(b);
+  return hip_check(c, hipGetLastError(), "batch step");
+}
+
+int hcs_batch_get(hcs_batch* b, hcs_pair_out* out) {
+  if (!b || !out) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  int rc = hip_check(c, hipMemcpyAsync(out, b->out.ptr, sizeof(PairOut) * b->npairs, hipMemcpyDeviceToHost, c->stream),
+                     "download");
+  if (rc) return rc;
+  rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  for (int e = 0; e < b->npairs; ++e)
+    if (out[e].status) return set_err(c, (int)out[e].status, "traversal stack overflow");
+  return HCS_OK;
+}
+
+int hcs_batch_counts(hcs_batch* b, int64_t* counts) {
+  if (!b || !counts) return HCS_ERR_ARG;
+  std::vector<hcs_pair_out> o(b->npairs);
+  int rc = hcs_batch_get(b, o.data());
+  counts[0] = counts[1] = counts[2] = 0;
+  for (auto& r : o) counts[0] += r.n_candidates, counts[1] += r.n_polygons, counts[2] += r.n_faces;
+  return rc;
+}
+
+int hcs_batch_config(hcs_batch* b, int32_t* cfg) {
+  if (!b || !cfg) return HCS_ERR_ARG;
+  cfg[0] = b->blocks_per_cu, cfg[1] = b->qpl, cfg[2] = b->tet_max;
+  return HCS_OK;
+}
+
+static int time_steps(hcs_batch* b, int reps, double* ms) {
+  Ctx* c = b->c;
+  hipEvent_t e0, e1;
+  int rc = hip_check(c, hipEventCreate(&e0), "event");
+  if (rc) return rc;
+  rc = hip_check(c, hipEventCreate(&e1), "event");
+  if (rc) return rc;
+  rc = hcs_batch_step(b); /* warm-up */
+  if (!rc) rc = hip_check(c, hipEventRecord(e0, c->stream), "record");
+  for (int r = 0; r < reps && !rc; ++r) rc = hcs_batch_step(b);
+  if (!rc) rc = hip_check(c, hipEventRecord(e1, c->stream), "record");
+  if (!rc) rc = hip_check(c, hipEventSynchronize(e1), "sync");
+  float t = 0;
+  if (!rc) rc = hip_check(c, hipEventElapsedTime(&t, e0, e1), "elapsed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = rc ? 0.0 : (double)t / std::max(1, reps);
+  return rc;
+}
+
+int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
+  if (!b || reps <= 0) return HCS_ERR_ARG;
+  const int bpc[] = {1, 2, 4, 8};
+  const int qpls[] = {1, 2};
+  const int tets[] = {16, 64, 256};
+  double best = 1e300;
+  int bb = b->blocks_per_cu, bq = b->qpl, bt = b->tet_max;
+  for (int t : tets)
+    for (int q : qpls) {
+      b->tet_max = t, b->qpl = q;
+      int rc = batch_plan(b);
+      if (rc) return rc;
+      for (int k : bpc) {
+        b->blocks_per_cu = k;
+        double ms = 0;
+        rc = time_steps(b, reps, &ms);
+        if (rc) return rc;
+        if (ms < best) best = ms, bb = k, bq = q, bt = t;
+      }
+    }
+  b->blocks_per_cu = bb, b->qpl = bq, b->tet_max = bt;
+  int rc = batch_plan(b);
+  if (!rc) rc = batch_capture(b);
+  if (best_ms) *best_ms = best;
+  return rc;
+}
+
+int hcs_batch_kernel_times(hcs_batch* b, int32_t reps, double* ms) {
+  if (!b || !ms || reps <= 0) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  hipStream_t s = c->stream;
+  hipEvent_t ev[5];
+  for (auto& e : ev) {
+    int rc = hip_check(c, hipEventCreate(&e), "event");
+    if (rc) return rc;
+  }
+  double acc[3] = {0, 0, 0};
+  int rc = HCS_OK;
+  for (int r = 0; r < reps && !rc; ++r) {
+    QueryArgs a = batch_args(b);
+    launch_fill_u32(s, a.counter, 0u, 1);
+    (void)hipEventRecord(ev[0], s);
+    launch_prep(s, a.envs, a.state, (double*)b->xf.ptr, b->npairs);
+    (void)hipEventRecord(ev[1], s);
+    launch_query(s, a, grid_of(b));
+    (void)hipEventRecord(ev[2], s);
+    launch_reduce(s, a.envs, a.partials, a.error, (PairOut*)b->out.ptr, b->npairs);
+    (void)hipEventRecord(ev[3], s);
+    rc = hip_check(c, hipEventSynchronize(ev[3]), "sync");
+    float t;
+    for (int k = 0; k < 3 && !rc; ++k) {
+      rc = hip_check(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]), "elapsed");
+      acc[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 3; ++k) ms[k] = acc[k] / reps;
+  return rc;
+}
+
+}  // extern "C"

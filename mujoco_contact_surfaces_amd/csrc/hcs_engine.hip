@@ -1,0 +1,822 @@
+/*
+ * hcs_engine.hip -- device kernels and host orchestration of the contact
+ * step: mesh upload + static LBVH build, the fused per-step query kernel,
+ * per-pair reduction, the plain per-call path and the batched persistent
+ * path captured in a hipGraph.
+ *
+ * Replaces, per step, what the reference does on the MuJoCo physics thread
+ * (citations relative to the reference repository):
+ *   collision_cb            mujoco_contact_surfaces/src/mujoco_contact_surfaces_plugin.cpp:255-318
+ *   Drake tet-tri / tet-tet queries                                  :284-304
+ *   evaluateContactSurface                                           :320-409
+ *   passiveCallback force loop                                       :411-483
+ * and at load time the Bvh<Obb, VolumeMesh> builds of
+ * parseMujocoCustomFields (:656,680,704,730,790).
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hcs_internal.h"
+#include "hcs_query.h"
+
+#pragma clang fp contract(off)
+
+namespace hcs {
+
+static thread_local std::string g_create_err;
+
+int set_err(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_check(Ctx* c, hipError_t e, const char* what) {
+  if (e == hipSuccess) return HCS_OK;
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  (void)hipGetLastError();
+  return set_err(c, e == hipErrorOutOfMemory ? HCS_ERR_OOM : HCS_ERR_HIP, m);
+}
+
+#define HCHK(c, expr)                                  \
+  do {                                                 \
+    int _rc = hip_check((c), (expr), #expr);           \
+    if (_rc != HCS_OK) return _rc;                     \
+  } while (0)
+
+int ensure(Ctx* c, DBuf& b, size_t bytes) {
+  if (bytes <= b.bytes && b.ptr) return HCS_OK;
+  if (b.ptr) {
+    HCHK(c, hipStreamSynchronize(c->stream));
+    HCHK(c, hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  size_t n = std::max<size_t>(bytes, 256);
+  HCHK(c, hipMalloc(&b.ptr, n));
+  b.bytes = n;
+  return HCS_OK;
+}
+
+/* ===================================================================
+ * Mesh preparation kernels (load time)
+ * =================================================================== */
+
+__constant__ int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+
+/* Tet record: face half-spaces, linear pressure field, AABB.  Same
+ * arithmetic as the oracle's tet record. */
+__global__ void k_tet_precompute(const double* __restrict__ V, const int32_t* __restrict__ T,
+                                 const double* __restrict__ p, int nt, double* __restrict__ rec,
+                                 double* __restrict__ box) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int32_t* tet = T + 4 * (size_t)t;
+  double v[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[i][c] = V[3 * (size_t)tet[i] + c];
+  double* r = rec + (size_t)HCS_TREC * t;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double* A = v[kFace[k][0]];
+    const double* B = v[kFace[k][1]];
+    const double* C = v[kFace[k][2]];
+    double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    double e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    double n[3];
+    cross3(e1, e2, n);
+    double len = sqrt(dot3(n, n));
+    double nh[3] = {n[0] / len, n[1] / len, n[2] / len};
+    r[3 * k] = nh[0], r[3 * k + 1] = nh[1], r[3 * k + 2] = nh[2];
+    r[12 + k] = -dot3(nh, A);
+  }
+  double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+  double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+  double e3[3] = {v[3][0] - v[0][0], v[3][1] - v[0][1], v[3][2] - v[0][2]};
+  double c23[3], c31[3], c12[3];
+  cross3(e2, e3, c23);
+  cross3(e3, e1, c31);
+  cross3(e1, e2, c12);
+  const double det = dot3(e1, c23);
+  const double p0 = p[tet[0]];
+  const double dp1 = p[tet[1]] - p0, dp2 = p[tet[2]] - p0, dp3 = p[tet[3]] - p0;
+  double g[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) g[i] = ((dp1 * c23[i] + dp2 * c31[i]) + dp3 * c12[i]) / det;
+  r[16] = g[0], r[17] = g[1], r[18] = g[2];
+  r[19] = p0 - dot3(g, v[0]);
+  const double gg = dot3(g, g);
+  if (gg > 0) {
+    const double gl = sqrt(gg);
+    r[20] = g[0] / gl, r[21] = g[1] / gl, r[22] = g[2] / gl;
+  } else {
+    r[20] = g[0], r[21] = g[1], r[22] = g[2];
+  }
+  r[23] = 0.0;
+  double* b = box + 6 * (size_t)t;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double lo = v[0][c], hi = v[0][c];
+#pragma unroll
+    for (int j = 1; j < 4; ++j) lo = fmin(lo, v[j][c]), hi = fmax(hi, v[j][c]);
+    b[c] = lo, b[3 + c] = hi;
+  }
+}
+
+/* Unit triangle normals (Drake's TriangleSurfaceMesh face normals). */
+__global__ void k_tri_normals(const double* __restrict__ V, const int32_t* __restrict__ F, int ntri,
+                              double* __restrict__ nrm) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= ntri) return;
+  const double* a = V + 3 * (size_t)F[3 * f];
+  const double* b = V + 3 * (size_t)F[3 * f + 1];
+  const double* c = V + 3 * (size_t)F[3 * f + 2];
+  double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  double n[3];
+  cross3(e1, e2, n);
+  const double z = dot3(n, n);
+  if (z > 0) {
+    const double l = sqrt(z);
+    n[0] /= l, n[1] /= l, n[2] /= l;
+  }
+  nrm[3 * f] = n[0], nrm[3 * f + 1] = n[1], nrm[3 * f + 2] = n[2];
+}
+
+__device__ inline unsigned expand_bits(unsigned v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void k_morton(const double* __restrict__ box, int n, double lx, double ly, double lz, double sx,
+                         double sy, double sz, unsigned* __restrict__ codes, int* __restrict__ idx) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double* b = box + 6 * (size_t)t;
+  double u[3] = {((b[0] + b[3]) * 0.5 - lx) * sx, ((b[1] + b[4]) * 0.5 - ly) * sy, ((b[2] + b[5]) * 0.5 - lz) * sz};
+  unsigned q[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) q[c] = (unsigned)fmin(fmax(u[c], 0.0), 1023.0);
+  codes[t] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+  idx[t] = t;
+}
+
+__device__ inline int delta(const unsigned* codes, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  unsigned a = codes[i], b = codes[j];
+  if (a == b) return 32 + __clz((unsigned)(i ^ j));
+  return __clz(a ^ b);
+}
+
+/* Karras radix-tree construction over sorted Morton codes: internal node i
+ * covers a key range whose split is the highest differing bit. */
+__global__ void k_karras(const unsigned* __restrict__ codes, const int* __restrict__ idx, int n,
+                         Node* __restrict__ nodes, int* __restrict__ node_parent, int* __restrict__ leaf_parent) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(codes, n, i, i - d);
+  int lmax = 2;
+  while (delta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(codes, n, i, j);
+  int s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (delta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + (d < 0 ? -1 : 0);
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  Node* N = nodes + i;
+  if (lo == gamma) {
+    N->c[0] = ~idx[gamma];
+    leaf_parent[gamma] = 2 * i;
+  } else {
+    N->c[0] = gamma;
+    node_parent[gamma] = 2 * i;
+  }
+  if (hi == gamma + 1) {
+    N->c[1] = ~idx[gamma + 1];
+    leaf_parent[gamma + 1] = 2 * i + 1;
+  } else {
+    N->c[1] = gamma + 1;
+    node_parent[gamma + 1] = 2 * i + 1;
+  }
+}
+
+/* Bottom-up refit: the second thread to reach a node writes its union box
+ * and height into the parent's child slot.  Counter RMWs are acq_rel at
+ * agent scope, which orders the slot stores/loads between the two threads. */
+__global__ void k_refit(const double* __restrict__ box, const int* __restrict__ idx, int n, Node* nodes,
+                        const int* __restrict__ node_parent, const int* __restrict__ leaf_parent, int* flags,
+                        int* root_height) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double* b = box + 6 * (size_t)idx[j];
+  int pc = leaf_parent[j];
+  int p = pc >> 1, side = pc & 1;
+  Node* N = nodes + p;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) N->b[6 * side + c] = b[c];
+  N->h[side] = 0;
+  for (int guard = 0; guard < n; ++guard) {
+    const int old = __hip_atomic_fetch_add(flags + p, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    N = nodes + p;
+    double u[6];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      u[c] = fmin(N->b[c], N->b[6 + c]);
+      u[3 + c] = fmax(N->b[3 + c], N->b[9 + c]);
+    }
+    const int h = 1 + (N->h[0] > N->h[1] ? N->h[0] : N->h[1]);
+    if (p == 0) {
+      *root_height = h;
+      return;
+    }
+    pc = node_parent[p];
+    p = pc >> 1, side = pc & 1;
+    Node* P = nodes + p;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) P->b[6 * side + c] = u[c];
+    P->h[side] = h;
+  }
+}
+
+/* ===================================================================
+ * Per-step kernels
+ * =================================================================== */
+
+__global__ void k_fill_u32(unsigned int* p, unsigned int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_prep(const EnvDesc* __restrict__ envs, const double* __restrict__ state, double* __restrict__ xf,
+                       int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double* st = state + (size_t)HCS_STATE * e;
+  double X[12];
+  compose_SR(st, st + 12, X);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) xf[12 * (size_t)e + i] = X[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_query(QueryArgs A, int dir_base, const double* dstar_dE, const double* invE,
+                                              Partial* tpart) {
+  __shared__ WaveLds L;
+  const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
+                  A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan};
+  const OutPtrs* op = (A.face_rec || A.poly_rec) ? &o : nullptr;
+  const int dir = dir_base < 0 ? -1 : dir_base + (int)blockIdx.y;
+  Partial* part = dir < 0 ? A.partials : tpart + (size_t)(dir - dir_base) * A.nitems;
+  if (A.counter == nullptr) {
+    const int it = blockIdx.x;
+    if (it < A.nitems) run_item<T>(L, A, it, dir, dstar_dE, invE, part + it, op);
+    return;
+  }
+  for (;;) {
+    unsigned v = 0;
+    if (threadIdx.x == 0) v = atomicAdd(A.counter + (blockIdx.y), 1u);
+    const int it = (int)__shfl(v, 0, 64);
+    if (it >= A.nitems) break;
+    run_item<T>(L, A, it, dir, dstar_dE, invE, part + it, op);
+  }
+}
+
+/* Per pair: partials of its items summed in item order. */
+__global__ void k_reduce(const EnvDesc* __restrict__ envs, const Partial* __restrict__ partials,
+                         const int* __restrict__ err, PairOut* __restrict__ out, int n) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int first = envs[e].item_first, cnt = envs[e].item_count;
+  double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
+  int64_t nc = 0, np = 0, nf = 0;
+  for (int i = first; i < first + cnt; ++i) {
+    const Partial& q = partials[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) F[c] = F[c] + q.F[c], tA[c] = tA[c] + q.tA[c], tB[c] = tB[c] + q.tB[c];
+    nc += q.nc, np += q.np, nf += q.nf;
+  }
+  PairOut r;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r.F_A[c] = F[c], r.tau_A[c] = tA[c], r.F_B[c] = -F[c], r.tau_B[c] = tB[c];
+  r.n_candidates = nc, r.n_polygons = np, r.n_faces = nf;
+  r.status = (err && *err) ? HCS_ERR_LIMIT : 0;
+  out[e] = r;
+}
+
+void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double* xf, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_prep, dim3((n + 127) / 128), dim3(128), 0, s, envs, state, xf, n);
+}
+void launch_query(hipStream_t s, const QueryArgs& a, int grid) {
+  if (grid > 0)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_query<double>), dim3(grid), dim3(64), 0, s, a, -1, (const double*)nullptr,
+                       (const double*)nullptr, (Partial*)nullptr);
+}
+void launch_query_tangent(hipStream_t s, const QueryArgs& a, int ndir, const double* dstar_dE, const double* invE,
+                          Partial* tpart) {
+  if (a.nitems > 0)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_query<Dual>), dim3(a.nitems, ndir), dim3(64), 0, s, a, 0, dstar_dE, invE,
+                       tpart);
+}
+void launch_query_tangent_dir(hipStream_t s, const QueryArgs& a, int dir, const double* dstar_dE,
+                              const double* invE, Partial* tpart) {
+  if (a.nitems > 0)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_query<Dual>), dim3(a.nitems, 1), dim3(64), 0, s, a, dir, dstar_dE, invE,
+                       tpart);
+}
+void launch_reduce(hipStream_t s, const EnvDesc* envs, const Partial* partials, const int* err, PairOut* out, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_reduce, dim3((n + 127) / 128), dim3(128), 0, s, envs, partials, err, out, n);
+}
+void launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+}
+
+/* ===================================================================
+ * Host side
+ * =================================================================== */
+
+static bool is_rigid(const hcs_props& p) {
+  return !(p.hydroelastic_modulus > 0) || std::isinf(p.hydroelastic_modulus);
+}
+
+static int upload(Ctx* c, void** dst, const void* src, size_t bytes) {
+  HCHK(c, hipMalloc(dst, std::max<size_t>(bytes, 8)));
+  if (bytes) HCHK(c, hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return HCS_OK;
+}
+
+static int build_bvh(Ctx* c, Mesh& m) {
+  const int n = m.nelem;
+  m.height = 0;
+  if (n == 1) {
+    HCHK(c, hipMalloc((void**)&m.nodes, sizeof(Node)));
+    return HCS_OK;
+  }
+  /* Scene bounds from the vertices (they bound every tet AABB). */
+  std::vector<double> Vh(3 * (size_t)m.nv);
+  HCHK(c, hipMemcpyAsync(Vh.data(), m.V, Vh.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int v = 0; v < m.nv; ++v)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], Vh[3 * v + k]), hi[k] = std::max(hi[k], Vh[3 * v + k]);
+  double sc[3];
+  for (int k = 0; k < 3; ++k) sc[k] = hi[k] > lo[k] ? 1024.0 / (hi[k] - lo[k]) : 0.0;
+  unsigned *codes = nullptr, *codes2 = nullptr;
+  int *idx = nullptr, *idx2 = nullptr, *np = nullptr, *lp = nullptr, *flags = nullptr, *rh = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  int rc = HCS_OK;
+  auto fail = [&](hipError_t e, const char* w) {
+    if (rc == HCS_OK) rc = hip_check(c, e, w);
+  };
+  fail(hipMalloc((void**)&codes, n * sizeof(unsigned)), "hipMalloc");
+  fail(hipMalloc((void**)&codes2, n * sizeof(unsigned)), "hipMalloc");
+  fail(hipMalloc((void**)&idx, n * sizeof(int)), "hipMalloc");
+  fail(hipMalloc((void**)&idx2, n * sizeof(int)), "hipMalloc");
+  fail(hipMalloc((void**)&np, n * sizeof(int)), "hipMalloc");
+  fail(hipMalloc((void**)&lp, n * sizeof(int)), "hipMalloc");
+  fail(hipMalloc((void**)&flags, n * sizeof(int)), "hipMalloc");
+  fail(hipMalloc((void**)&rh, sizeof(int)), "hipMalloc");
+  fail(hipMalloc((void**)&m.nodes, (size_t)(n - 1) * sizeof(Node)), "hipMalloc");
+  if (rc == HCS_OK) {
+    hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, m.box, n, lo[0], lo[1], lo[2],
+                       sc[0], sc[1], sc[2], codes, idx);
+    fail(hipGetLastError(), "k_morton");
+    fail(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes, codes2, idx, idx2, n, 0, 30, c->stream),
+         "SortPairs");
+    fail(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 8)), "hipMalloc");
+  }
+  if (rc == HCS_OK) {
+    fail(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, codes, codes2, idx, idx2, n, 0, 30, c->stream),
+         "SortPairs");
+    fail(hipMemsetAsync(flags, 0, n * sizeof(int), c->stream), "memset");
+    hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, c->stream, codes2, idx2, n, m.nodes, np,
+                       lp);
+    fail(hipGetLastError(), "k_karras");
+    hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, c->stream, m.box, idx2, n, m.nodes, np, lp,
+                       flags, rh);
+    fail(hipGetLastError(), "k_refit");
+    int h = 0;
+    fail(hipMemcpyAsync(&h, rh, sizeof(int), hipMemcpyDeviceToHost, c->stream), "memcpy");
+    fail(hipStreamSynchronize(c->stream), "sync");
+    m.height = h;
+  }
+  for (void* ptr : {(void*)codes, (void*)codes2, (void*)idx, (void*)idx2, (void*)np, (void*)lp, (void*)flags,
+                    (void*)rh, tmp})
+    if (ptr) (void)hipFree(ptr);
+  if (rc == HCS_OK && m.height > HCS_STACK - 2)
+    rc = set_err(c, HCS_ERR_LIMIT, "LBVH height " + std::to_string(m.height) + " exceeds the traversal stack");
+  return rc;
+}
+
+static void free_mesh(Mesh& m) {
+  for (void* p : {(void*)m.V, (void*)m.E, (void*)m.p, (void*)m.rec, (void*)m.box, (void*)m.nodes, (void*)m.nrm})
+    if (p) (void)hipFree(p);
+  m = Mesh();
+}
+
+int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int qpl, int tet_max, Plan& plan) {
+  plan.envs.assign(npairs, EnvDesc{});
+  plan.items.clear();
+  plan.pair_kind.assign(npairs, 0);
+  const int per_item = 64 * std::max(1, qpl);
+  for (int e = 0; e < npairs; ++e) {
+    const int ha = geoms[2 * e], hb = geoms[2 * e + 1];
+    if (ha < 0 || hb < 0 || ha >= (int)c->meshes.size() || hb >= (int)c->meshes.size() ||
+        c->meshes[ha].kind == 0 || c->meshes[hb].kind == 0)
+      return set_err(c, HCS_ERR_ARG, "pair " + std::to_string(e) + ": invalid mesh handle");
+    const Mesh& A = c->meshes[ha];
+    const Mesh& B = c->meshes[hb];
+    if (A.kind != 1)
+      return set_err(c, HCS_ERR_ARG,
+                     "pair " + std::to_string(e) + ": geom_a must be soft (pass soft-rigid pairs soft first)");
+    EnvDesc& E = plan.envs[e];
+    E.recA = A.rec, E.boxA = A.box, E.nodesA = A.nodes, E.TA = A.E, E.VA = A.V, E.ntA = A.nelem;
+    E.VB = B.V, E.FB = B.E, E.nrmB = B.nrm, E.recB = B.rec, E.nB = B.nelem, E.nvB = B.nv;
+    E.b_soft = B.kind == 1;
+    E.a_is_M = A.geom_id < B.geom_id;
+    E.repr = o ? o->representation : 0;
+    E.apply = o ? o->apply_forces : 1;
+    E.dt = o ? o->dt : 0.001;
+    const double inf = std::numeric_limits<double>::infinity();
+    const double EA = A.props.hydroelastic_modulus, dA = A.props.dissipation;
+    const double EB = E.b_soft ? B.props.hydroelastic_modulus : inf, dB = E.b_soft ? B.props.dissipation : 1.0;
+    /* calcCombinedDissipation (plugin.cpp:138-159) */
+    {
+      double Estar = (EA == inf) ? EB : (EB == inf ? EA : EA * EB / (EA + EB));
+      double ds;
+      if (Estar == inf) {
+        ds = 0.5 * (dA + dB);
+      } else {
+        ds = 0;
+        if (EA != inf) ds += Estar / EA * dA;
+        if (EB != inf) ds += Estar / EB * dB;
+      }
+      E.dstar = ds;
+    }
+    /* CalcContactFrictionFromSurfaceProperties, dynamic coefficient (plugin.cpp:427-433) */
+    {
+      const double m1 = A.props.dynamic_friction, m2 = B.props.dynamic_friction;
+      const double den = m1 + m2;
+      E.mu = den == 0.0 ? 0.0 : 2 * m1 * m2 / den;
+    }
+    int nq, chunk;
+    if (E.b_soft) {
+      E.kind = KIND_SOFT;
+      nq = B.nelem;
+      chunk = per_item;
+    } else if (B.nelem <= tet_max) {
+      E.kind = KIND_TET;
+      nq = A.nelem;
+      chunk = 64;
+    } else {
+      E.kind = KIND_TRI;
+      nq = B.nelem;
+      chunk = per_item;
+    }
+    plan.pair_kind[e] = E.kind;
+    E.item_first = (int)plan.items.size();
+    for (int q = 0; q < nq; q += chunk) plan.items.push_back(Item{e, q, std::min(chunk, nq - q), 0});
+    E.item_count = (int)plan.items.size() - E.item_first;
+  }
+  return HCS_OK;
+}
+
+}  // namespace hcs
+
+using namespace hcs;
+
+/* ===================================================================
+ * C-ABI
+ * =================================================================== */
+
+extern "C" {
+
+int hcs_create(hcs_ctx** out, int device) {
+  if (!out) return HCS_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    g_create_err = "no HIP device available";
+    (void)hipGetLastError();
+    return HCS_ERR_HIP;
+  }
+  if (device < 0 || device >= count) {
+    g_create_err = "device ordinal out of range";
+    return HCS_ERR_ARG;
+  }
+  hcs_ctx* h = new (std::nothrow) hcs_ctx();
+  if (!h) return HCS_ERR_OOM;
+  Ctx* c = &h->c;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    g_create_err = "hipSetDevice/hipStreamCreate failed";
+    delete h;
+    return HCS_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+  if (c->num_cu <= 0) c->num_cu = 256;
+  *out = h;
+  return HCS_OK;
+}
+
+int hcs_destroy(hcs_ctx* h) {
+  if (!h) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& m : c->meshes) free_mesh(m);
+  for (DBuf* b : {&c->d_envs, &c->d_state, &c->d_xf, &c->d_items, &c->d_partials, &c->d_out, &c->d_err,
+                  &c->d_face_rec, &c->d_face_key, &c->d_face_cnt, &c->d_poly_rec, &c->d_poly_key, &c->d_poly_cnt,
+                  &c->d_cand, &c->d_cand_cnt, &c->d_tx_cnt, &c->d_tx_off, &c->d_tx_list, &c->d_tx_out,
+                  &c->d_tx_grid})
+    if (b->ptr) (void)hipFree(b->ptr);
+  (void)hipStreamDestroy(c->stream);
+  delete h;
+  return HCS_OK;
+}
+
+const char* hcs_last_error(const hcs_ctx* h) { return h ? h->c.err.c_str() : g_create_err.c_str(); }
+
+int hcs_synchronize(hcs_ctx* h) {
+  if (!h) return HCS_ERR_ARG;
+  return hip_check(&h->c, hipStreamSynchronize(h->c.stream), "hipStreamSynchronize");
+}
+
+void* hcs_stream(hcs_ctx* h) { return h ? (void*)h->c.stream : nullptr; }
+
+int hcs_add_soft_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* T, int32_t nt, const double* pr,
+                      const hcs_props* props, int32_t geom_id, int32_t* handle) {
+  if (!h || !V || !T || !pr || !props || !handle || nv < 4 || nt < 1) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!(props->hydroelastic_modulus > 0) || std::isinf(props->hydroelastic_modulus))
+    return set_err(c, HCS_ERR_ARG, "soft mesh needs a finite hydroelastic_modulus > 0");
+  for (int64_t i = 0; i < 4 * (int64_t)nt; ++i)
+    if (T[i] < 0 || T[i] >= nv) return set_err(c, HCS_ERR_ARG, "tet vertex index out of range");
+  HCHK(c, hipSetDevice(c->device));
+  Mesh m;
+  m.kind = 1;
+  m.geom_id = geom_id;
+  m.props = *props;
+  m.nv = nv;
+  m.nelem = nt;
+  int rc = upload(c, (void**)&m.V, V, 3 * sizeof(double) * nv);
+  if (rc == HCS_OK) rc = upload(c, (void**)&m.E, T, 4 * sizeof(int32_t) * nt);
+  if (rc == HCS_OK) rc = upload(c, (void**)&m.p, pr, sizeof(double) * nv);
+  if (rc == HCS_OK) rc = hip_check(c, hipMalloc((void**)&m.rec, (size_t)HCS_TREC * sizeof(double) * nt), "hipMalloc");
+  if (rc == HCS_OK) rc = hip_check(c, hipMalloc((void**)&m.box, 6 * sizeof(double) * (size_t)nt), "hipMalloc");
+  if (rc == HCS_OK) {
+    hipLaunchKernelGGL(k_tet_precompute, dim3((nt + 127) / 128), dim3(128), 0, c->stream, m.V, m.E, m.p, nt, m.rec,
+                       m.box);
+    rc = hip_check(c, hipGetLastError(), "k_tet_precompute");
+  }
+  if (rc == HCS_OK) rc = build_bvh(c, m);
+  if (rc != HCS_OK) {
+    (void)hipStreamSynchronize(c->stream);
+    free_mesh(m);
+    return rc;
+  }
+  *handle = (int32_t)c->meshes.size();
+  c->meshes.push_back(m);
+  return HCS_OK;
+}
+
+int hcs_add_rigid_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* F, int32_t ntri,
+                       const hcs_props* props, int32_t geom_id, int32_t* handle) {
+  if (!h || !V || !F || !props || !handle || nv < 3 || ntri < 1) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!is_rigid(*props)) return set_err(c, HCS_ERR_ARG, "rigid mesh needs hydroelastic_modulus 0 or +inf");
+  for (int64_t i = 0; i < 3 * (int64_t)ntri; ++i)
+    if (F[i] < 0 || F[i] >= nv) return set_err(c, HCS_ERR_ARG, "triangle vertex index out of range");
+  HCHK(c, hipSetDevice(c->device));
+  Mesh m;
+  m.kind = 2;
+  m.geom_id = geom_id;
+  m.props = *props;
+  m.nv = nv;
+  m.nelem = ntri;
+  int rc = upload(c, (void**)&m.V, V, 3 * sizeof(double) * nv);
+  if (rc == HCS_OK) rc = upload(c, (void**)&m.E, F, 3 * sizeof(int32_t) * ntri);
+  if (rc == HCS_OK) rc = hip_check(c, hipMalloc((void**)&m.nrm, 3 * sizeof(double) * (size_t)ntri), "hipMalloc");
+  if (rc == HCS_OK) {
+    hipLaunchKernelGGL(k_tri_normals, dim3((ntri + 127) / 128), dim3(128), 0, c->stream, m.V, m.E, ntri, m.nrm);
+    rc = hip_check(c, hipGetLastError(), "k_tri_normals");
+  }
+  if (rc == HCS_OK) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc != HCS_OK) {
+    free_mesh(m);
+    return rc;
+  }
+  *handle = (int32_t)c->meshes.size();
+  c->meshes.push_back(m);
+  return HCS_OK;
+}
+
+int hcs_mesh_info(hcs_ctx* h, int32_t handle, int64_t* info) {
+  if (!h || !info || handle < 0 || handle >= (int)h->c.meshes.size()) return HCS_ERR_ARG;
+  const Mesh& m = h->c.meshes[handle];
+  info[0] = m.kind, info[1] = m.nv, info[2] = m.nelem, info[3] = m.height;
+  return HCS_OK;
+}
+
+int hcs_mesh_tet_records(hcs_ctx* h, int32_t handle, double* out) {
+  if (!h || !out || handle < 0 || handle >= (int)h->c.meshes.size() || h->c.meshes[handle].kind != 1)
+    return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  const Mesh& m = c->meshes[handle];
+  HCHK(c, hipMemcpyAsync(out, m.rec, (size_t)HCS_TREC * sizeof(double) * m.nelem, hipMemcpyDeviceToHost, c->stream));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  return HCS_OK;
+}
+
+}  // extern "C"
+
+namespace hcs {
+
+/* Shared body of hcs_contact_step / hcs_candidates: plan, upload, query,
+ * reduce.  keys: candidate-key mode. */
+static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, hcs_pair_out* out,
+                     hcs_faces* faces, int64_t* keys, int64_t key_cap, int64_t* key_count) {
+  HCHK(c, hipSetDevice(c->device));
+  std::vector<int32_t> geoms(2 * (size_t)npairs);
+  std::vector<double> state((size_t)HCS_STATE * npairs);
+  for (int e = 0; e < npairs; ++e) {
+    geoms[2 * e] = in[e].geom_a, geoms[2 * e + 1] = in[e].geom_b;
+    double* s = &state[(size_t)HCS_STATE * e];
+    std::memcpy(s, in[e].X_WA, 12 * sizeof(double));
+    std::memcpy(s + 12, in[e].X_WB, 12 * sizeof(double));
+    std::memcpy(s + 24, in[e].V_WA, 6 * sizeof(double));
+    std::memcpy(s + 30, in[e].V_WB, 6 * sizeof(double));
+  }
+  Plan plan;
+  int rc = build_plan(c, npairs, geoms.data(), opts, 1, c->tet_mode_max_tris, plan);
+  if (rc != HCS_OK) return rc;
+  const int nitems = (int)plan.items.size();
+  if ((rc = ensure(c, c->d_envs, sizeof(EnvDesc) * npairs)) || (rc = ensure(c, c->d_state, state.size() * 8)) ||
+      (rc = ensure(c, c->d_xf, 12 * sizeof(double) * npairs)) ||
+      (rc = ensure(c, c->d_items, sizeof(Item) * std::max(1, nitems))) ||
+      (rc = ensure(c, c->d_partials, sizeof(Partial) * std::max(1, nitems))) ||
+      (rc = ensure(c, c->d_out, sizeof(PairOut) * npairs)) || (rc = ensure(c, c->d_err, 64)) ||
+      (rc = ensure(c, c->d_face_cnt, 64)) || (rc = ensure(c, c->d_poly_cnt, 64)) ||
+      (rc = ensure(c, c->d_cand_cnt, 64)))
+    return rc;
+  const bool keep = opts && opts->keep_polygons && !keys;
+  const bool want_faces = faces && faces->capacity > 0 && faces->rec && faces->key && !keys;
+  if (want_faces) {
+    if ((rc = ensure(c, c->d_face_rec, sizeof(double) * HCS_FREC * faces->capacity)) ||
+        (rc = ensure(c, c->d_face_key, sizeof(int64_t) * 4 * faces->capacity)))
+      return rc;
+  }
+  if (keys && (rc = ensure(c, c->d_cand, sizeof(int64_t) * std::max<int64_t>(key_cap, 1)))) return rc;
+  int64_t poly_cap = 0;
+  if (keep) {
+    /* capacity grows until it holds the step's polygons (rerun below) */
+    poly_cap = std::max<int64_t>(c->d_poly_rec.bytes / (sizeof(double) * HCS_PREC), 4096);
+    if ((rc = ensure(c, c->d_poly_rec, sizeof(double) * HCS_PREC * poly_cap)) ||
+        (rc = ensure(c, c->d_poly_key, sizeof(int64_t) * 3 * poly_cap)))
+      return rc;
+  }
+  HCHK(c, hipMemcpyAsync(c->d_envs.ptr, plan.envs.data(), sizeof(EnvDesc) * npairs, hipMemcpyHostToDevice,
+                         c->stream));
+  HCHK(c, hipMemcpyAsync(c->d_state.ptr, state.data(), state.size() * 8, hipMemcpyHostToDevice, c->stream));
+  if (nitems)
+    HCHK(c, hipMemcpyAsync(c->d_items.ptr, plan.items.data(), sizeof(Item) * nitems, hipMemcpyHostToDevice,
+                           c->stream));
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    HCHK(c, hipMemsetAsync(c->d_err.ptr, 0, 64, c->stream));
+    HCHK(c, hipMemsetAsync(c->d_face_cnt.ptr, 0, 64, c->stream));
+    HCHK(c, hipMemsetAsync(c->d_poly_cnt.ptr, 0, 64, c->stream));
+    HCHK(c, hipMemsetAsync(c->d_cand_cnt.ptr, 0, 64, c->stream));
+    QueryArgs a{};
+    a.envs = (const EnvDesc*)c->d_envs.ptr;
+    a.state = (const double*)c->d_state.ptr;
+    a.xf = (const double*)c->d_xf.ptr;
+    a.items = (const Item*)c->d_items.ptr;
+    a.nitems = nitems;
+    a.queries_per_lane = 1;
+    a.counter = nullptr;
+    a.partials = (Partial*)c->d_partials.ptr;
+    if (want_faces) {
+      a.face_rec = (double*)c->d_face_rec.ptr;
+      a.face_key = (int64_t*)c->d_face_key.ptr;
+      a.face_cnt = (unsigned long long*)c->d_face_cnt.ptr;
+      a.face_cap = faces->capacity;
+    }
+    if (keep) {
+      a.poly_rec = (double*)c->d_poly_rec.ptr;
+      a.poly_key = (int64_t*)c->d_poly_key.ptr;
+      a.poly_cnt = (unsigned long long*)c->d_poly_cnt.ptr;
+      a.poly_cap = poly_cap;
+    }
+    if (keys) {
+      a.cand_key = (int64_t*)c->d_cand.ptr;
+      a.cand_cnt = (unsigned long long*)c->d_cand_cnt.ptr;
+      a.cand_cap = key_cap;
+    }
+    a.error = (int*)c->d_err.ptr;
+    launch_prep(c->stream, a.envs, a.state, (double*)c->d_xf.ptr, npairs);
+    launch_query(c->stream, a, nitems);
+    HCHK(c, hipGetLastError());
+    launch_reduce(c->stream, a.envs, a.partials, a.error, (PairOut*)c->d_out.ptr, npairs);
+    HCHK(c, hipGetLastError());
+    unsigned long long npoly = 0;
+    if (keep) {
+      HCHK(c, hipMemcpyAsync(&npoly, c->d_poly_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+      HCHK(c, hipStreamSynchronize(c->stream));
+      if ((int64_t)npoly > poly_cap && attempt == 0) {
+        poly_cap = (int64_t)npoly + npoly / 4 + 1024;
+        if ((rc = ensure(c, c->d_poly_rec, sizeof(double) * HCS_PREC * poly_cap)) ||
+            (rc = ensure(c, c->d_poly_key, sizeof(int64_t) * 3 * poly_cap)))
+          return rc;
+        continue;
+      }
+      c->last_npoly = (int64_t)npoly;
+    }
+    break;
+  }
+  std::vector<PairOut> res(npairs);
+  HCHK(c, hipMemcpyAsync(res.data(), c->d_out.ptr, sizeof(PairOut) * npairs, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long nface = 0, ncand = 0;
+  HCHK(c, hipMemcpyAsync(&nface, c->d_face_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+  HCHK(c, hipMemcpyAsync(&ncand, c->d_cand_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+  HCHK(c, hipStreamSynchronize(c->stream));
+  if (want_faces) {
+    const int64_t nw = std::min<int64_t>((int64_t)nface, faces->capacity);
+    if (nw > 0) {
+      HCHK(c, hipMemcpyAsync(faces->rec, c->d_face_rec.ptr, sizeof(double) * HCS_FREC * nw, hipMemcpyDeviceToHost,
+                             c->stream));
+      HCHK(c, hipMemcpyAsync(faces->key, c->d_face_key.ptr, sizeof(int64_t) * 4 * nw, hipMemcpyDeviceToHost,
+                             c->stream));
+      HCHK(c, hipStreamSynchronize(c->stream));
+    }
+    faces->count = (int64_t)nface;
+  } else if (faces) {
+    faces->count = 0;
+  }
+  if (keys) {
+    const int64_t nw = std::min<int64_t>((int64_t)ncand, key_cap);
+    if (nw > 0) {
+      HCHK(c, hipMemcpyAsync(keys, c->d_cand.ptr, sizeof(int64_t) * nw, hipMemcpyDeviceToHost, c->stream));
+      HCHK(c, hipStreamSynchronize(c->stream));
+      std::sort(keys, keys + nw);
+    }
+    if (key_count) *key_count = (int64_t)ncand;
+  }
+  if (out) std::memcpy(out, res.data(), sizeof(PairOut) * npairs);
+  int status = HCS_OK;
+  for (auto& r : res)
+    if (r.status) status = (int)r.status;
+  if (keep) {
+    c->last_valid = 1;
+    c->last_pairs.assign(in, in + npairs);
+    c->last_geoms = geoms;
+  }
+  if (status == HCS_ERR_LIMIT) return set_err(c, status, "traversal stack overflow");
+  if (want_faces && (int64_t)nface > faces->capacity)
+    return set_err(c, HCS_ERR_CAPACITY, "face buffer too small: " + std::to_string(nface) + " faces");
+  if (keys && (int64_t)ncand > key_cap)
+    return set_err(c, HCS_ERR_CAPACITY, "key buffer too small: " + std::to_string(ncand) + " keys");
+  return HCS_OK;
+}
+
+}  // namespace hcs
+
+extern "C" {
+
+int hcs_contact_step(hcs_ctx* h, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts,
+                     hcs_pair_out* out, hcs_faces* faces) {
+  if (!h || npairs < 0 || (npairs > 0 && (!in || !out))) return HCS_ERR_ARG;
+  if (npairs == 0) return HCS_OK;
+  static_assert(sizeof(hcs_pair_out) == sizeof(PairOut), "pair_out layout");
+  return run_pairs(&h->c, npairs, in, opts, out, faces, nullptr, 0, nullptr);
+}
+
+int hcs_candidates(hcs_ctx* h, const hcs_pair_in* pair, int64_t* keys, int64_t capacity, int64_t* count) {
+  if (!h || !pair || !count || capacity < 0 || (capacity > 0 && !keys)) return HCS_ERR_ARG;
+  int64_t dummy = 0;
+  return run_pairs(&h->c, 1, pair, nullptr, nullptr, nullptr, keys ? keys : &dummy, capacity, count);
+}
+
+}  // extern "C"

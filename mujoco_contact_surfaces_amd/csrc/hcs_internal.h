@@ -1,0 +1,98 @@
+/*
+ * hcs_internal.h -- host-side context shared by the engine translation units.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hcs.h"
+#include "hcs_kernels.h"
+
+namespace hcs {
+
+struct Mesh {
+  int kind = 0; /* 1 soft, 2 rigid */
+  int32_t geom_id = 0;
+  hcs_props props{};
+  int32_t nv = 0, nelem = 0, height = 0;
+  double* V = nullptr;
+  int32_t* E = nullptr; /* T (soft) or F (rigid) */
+  double* p = nullptr;
+  double* rec = nullptr;
+  double* box = nullptr;
+  Node* nodes = nullptr;
+  double* nrm = nullptr;
+};
+
+/* Grow-only device buffer. */
+struct DBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct Ctx;
+int set_err(Ctx* c, int code, const std::string& msg);
+int hip_check(Ctx* c, hipError_t e, const char* what);
+int ensure(Ctx* c, DBuf& b, size_t bytes);
+
+/* Per-call plan: env descriptors and items for a set of pairs. */
+struct Plan {
+  std::vector<EnvDesc> envs;
+  std::vector<Item> items;
+  std::vector<int> pair_kind;
+};
+
+struct Ctx {
+  int device = 0;
+  int num_cu = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<Mesh> meshes;
+  /* scratch for hcs_contact_step / hcs_candidates / hcs_taxels */
+  DBuf d_envs, d_state, d_xf, d_items, d_partials, d_out, d_err;
+  DBuf d_face_rec, d_face_key, d_face_cnt;
+  DBuf d_poly_rec, d_poly_key, d_poly_cnt;
+  DBuf d_cand, d_cand_cnt;
+  DBuf d_tx_cnt, d_tx_off, d_tx_list, d_tx_out, d_tx_grid;
+  /* last contact step's polygons (for taxels) */
+  int64_t last_npoly = 0;
+  int last_valid = 0;
+  std::vector<hcs_pair_in> last_pairs;
+  std::vector<int32_t> last_geoms; /* per pair: geom_a, geom_b */
+  int tet_mode_max_tris = 64;
+};
+
+/* Builds env descriptors and work items; tri-query chunks of 64 *
+ * queries_per_lane queries.  Returns HCS_OK or an error. */
+int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int queries_per_lane,
+               int tet_mode_max_tris, Plan& plan);
+
+/* Launchers (hcs_engine.hip). */
+void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double* xf, int n);
+void launch_query(hipStream_t s, const QueryArgs& a, int grid);
+void launch_reduce(hipStream_t s, const EnvDesc* envs, const Partial* partials, const int* err, PairOut* out,
+                   int n);
+void launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, int n);
+void launch_query_tangent_dir(hipStream_t s, const QueryArgs& a, int dir, const double* dstar_dE,
+                              const double* invE, Partial* tpart);
+void launch_query_tangent(hipStream_t s, const QueryArgs& a, int ndir, const double* dstar_dE, const double* invE,
+                          Partial* tpart);
+
+struct TaxelGridDev {
+  double X[12];
+  double xs, ys, zs, res, sigma;
+  int S, window, cx, cy;
+};
+int taxel_grid_dev(const hcs_taxel_grid& G, TaxelGridDev& g);
+int run_taxels(Ctx* c, const TaxelGridDev& g, const unsigned char* d_mask, const double* poly, const double* ptan,
+               const int64_t* key, int64_t np, const double* Xd, double* d_out);
+/* Uploads the plan/state of npairs pairs into the context scratch buffers. */
+int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan);
+
+}  // namespace hcs
+
+struct hcs_ctx {
+  hcs::Ctx c;
+};

@@ -1,0 +1,93 @@
+/*
+ * hcs_kernels.h -- device data layout shared by the engine's kernels and its
+ * host orchestration (hcs_engine.hip, hcs_taxels.hip, hcs_adjoint.hip).
+ *
+ * HBM layout (all FP64 unless noted; one set per mesh, allocated at load):
+ *   soft mesh:  rec   nt x 24  tet records (4 unit outward face normals,
+ *                               4 plane offsets, pressure gradient g,
+ *                               constant c with p(x) = c + g.x, unit g, pad)
+ *               box   nt x 6   AABB lo, hi (soft frame)
+ *               nodes (nt-1) x Node   binary LBVH, child boxes stored in the
+ *                               parent so one 112-byte fetch tests both
+ *               T     nt x 4 int32, V nv x 3, p nv
+ *   rigid mesh: V nv x 3, F ntri x 3 int32, nrm ntri x 3 (unit normals)
+ * Per step:    state  npairs x 36 (X_WA, X_WB, V_WA, V_WB), xf npairs x 12
+ *              (X_AB = X_WA^-1 X_WB), items, per-item partial wrenches,
+ *              per-pair outputs.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hcs {
+
+#define HCS_TREC 24
+#define HCS_STACK 64
+#define HCS_FREC 12
+#define HCS_PREC 40
+#define HCS_STATE 36
+
+struct Node {
+  double b[12];  /* child 0 lo[3] hi[3], child 1 lo[3] hi[3] */
+  int32_t c[2];  /* >= 0 internal node, < 0 leaf: ~element */
+  int32_t h[2];  /* subtree heights (leaf 0) */
+};
+
+enum EnvKind { KIND_TRI = 0, KIND_TET = 1, KIND_SOFT = 2 };
+
+struct EnvDesc {
+  const double* recA;
+  const double* boxA;
+  const Node* nodesA;
+  const int32_t* TA;
+  const double* VA;
+  const double* VB;
+  const int32_t* FB;  /* rigid B triangles, or soft B tets */
+  const double* nrmB; /* rigid B normals */
+  const double* recB; /* soft B records */
+  int32_t ntA, nB, kind, nvB;
+  int32_t a_is_M, repr, apply, b_soft;
+  double mu, dstar, dt;
+  int32_t item_first, item_count;
+  int32_t pad[2];
+};
+
+struct Item {
+  int32_t env, first, count, pad;
+};
+
+struct Partial {
+  double F[3], tA[3], tB[3];
+  int64_t nc, np, nf;
+};
+
+struct PairOut {
+  double F_A[3], tau_A[3], F_B[3], tau_B[3];
+  int64_t n_candidates, n_polygons, n_faces, status;
+};
+
+struct QueryArgs {
+  const EnvDesc* envs;
+  const double* state;
+  const double* xf;
+  const Item* items;
+  int32_t nitems;
+  int32_t queries_per_lane;
+  unsigned int* counter; /* persistent work queue; nullptr: item = blockIdx.x */
+  Partial* partials;
+  double* face_rec;
+  int64_t* face_key;
+  unsigned long long* face_cnt;
+  int64_t face_cap;
+  double* poly_rec;
+  int64_t* poly_key;
+  unsigned long long* poly_cnt;
+  int64_t poly_cap;
+  double* poly_tan; /* tangent copy of poly_rec (tangent pass only) */
+  int64_t* cand_key; /* keys-only mode when non-null */
+  unsigned long long* cand_cnt;
+  int64_t cand_cap;
+  int* error;
+};
+
+}  // namespace hcs

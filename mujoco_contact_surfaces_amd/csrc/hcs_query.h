@@ -1,0 +1,589 @@
+/*
+ * hcs_query.h -- the fused per-step contact kernel body (device code),
+ * templated on the scalar type: double for the forward step, Dual for the
+ * tangent-linear backward (one tangent direction per grid.y slice).
+ *
+ * One wavefront owns one work item (a chunk of queries of one pair):
+ *   1. each lane builds the AABB of its query element in the soft frame A
+ *      (a rigid triangle moved by X_AB, a tet of soft body B, or a tet of A
+ *      when the rigid side is tiny and is brute-forced from LDS);
+ *   2. lanes walk A's LBVH (stack in LDS); leaf hits become candidate pairs
+ *      that are compacted into an LDS queue with a wave ballot + mbcnt
+ *      prefix count;
+ *   3. whenever 64 candidates are queued, every lane takes one: clip
+ *      (registers only), faces, force law, wrench accumulation;
+ *   4. the item's wrench is reduced across the wave in a fixed butterfly
+ *      order and stored at partials[item], so results do not depend on which
+ *      wave ran the item or in what order items were taken.
+ * Reference semantics per candidate: see hcs_device.h.
+ */
+#pragma once
+#include "hcs_device.h"
+#include "hcs_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace hcs {
+
+/* Per-item uniform inputs. */
+template <class T>
+struct ItemCtx {
+  PairW<T> pw;
+  T X_AB[12];
+  T escale; /* scales A's pressure field (E tangent); 1 in the forward pass */
+  int repr;
+};
+
+template <class T>
+__device__ inline T seed_val(double v) {
+  return T(v);
+}
+
+/* Builds the per-item context.  dir < 0: forward; 0..2 rotation of B about
+ * world axis dir, 3..5 translation of B along world axis dir-3, 6 modulus of
+ * A (tangents 1/E_A on A's field). */
+template <class T>
+__device__ inline void make_item_ctx(const EnvDesc& E, const double* st, int dir, double dstar_dE, double invE,
+                                     ItemCtx<T>& c);
+
+template <>
+__device__ inline void make_item_ctx<double>(const EnvDesc& E, const double* st, int, double, double,
+                                             ItemCtx<double>& c) {
+#pragma unroll
+  for (int i = 0; i < 12; ++i) c.pw.X_WA[i] = st[i], c.pw.X_WB[i] = st[12 + i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) c.pw.V_WA[i] = st[24 + i], c.pw.V_WB[i] = st[30 + i];
+  c.pw.mu = E.mu, c.pw.dstar = E.dstar, c.pw.dt = E.dt;
+  c.pw.apply = E.apply, c.pw.a_is_M = E.a_is_M, c.pw.b_soft = E.b_soft;
+  compose_SR(c.pw.X_WA, c.pw.X_WB, c.X_AB);
+  c.escale = 1.0;
+  c.repr = E.repr;
+}
+
+template <>
+__device__ inline void make_item_ctx<Dual>(const EnvDesc& E, const double* st, int dir, double dstar_dE, double invE,
+                                           ItemCtx<Dual>& c) {
+#pragma unroll
+  for (int i = 0; i < 12; ++i) c.pw.X_WA[i] = Dual(st[i]), c.pw.X_WB[i] = Dual(st[12 + i]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) c.pw.V_WA[i] = Dual(st[24 + i]), c.pw.V_WB[i] = Dual(st[30 + i]);
+  if (dir >= 0 && dir < 3) {
+    /* d/dtheta of exp([theta]) R at 0 is [e_dir]x R, row by row. */
+    const double* R = st + 12;
+    for (int j = 0; j < 3; ++j) {
+      double r0 = R[j], r1 = R[3 + j], r2 = R[6 + j];
+      double d0 = 0, d1 = 0, d2 = 0;
+      if (dir == 0) d1 = -r2, d2 = r1;
+      if (dir == 1) d0 = r2, d2 = -r0;
+      if (dir == 2) d0 = -r1, d1 = r0;
+      c.pw.X_WB[j].d = d0, c.pw.X_WB[3 + j].d = d1, c.pw.X_WB[6 + j].d = d2;
+    }
+  } else if (dir >= 3 && dir < 6) {
+    c.pw.X_WB[9 + (dir - 3)].d = 1.0;
+  }
+  c.pw.mu = E.mu, c.pw.dt = E.dt;
+  c.pw.dstar = Dual(E.dstar, dir == 6 ? dstar_dE : 0.0);
+  c.pw.apply = E.apply, c.pw.a_is_M = E.a_is_M, c.pw.b_soft = E.b_soft;
+  compose_SR(c.pw.X_WA, c.pw.X_WB, c.X_AB);
+  c.escale = Dual(1.0, dir == 6 ? invE : 0.0);
+  c.repr = E.repr;
+}
+
+struct OutPtrs {
+  double* face_rec;
+  int64_t* face_key;
+  unsigned long long* face_cnt;
+  int64_t face_cap;
+  double* poly_rec;
+  int64_t* poly_key;
+  unsigned long long* poly_cnt;
+  int64_t poly_cap;
+  double* poly_tan;
+};
+
+/* Polygon (A frame) -> faces -> point collisions -> wrench; kTriangle fan
+ * (v_k, v_{k+1}, C) with the field at C, or one kPolygon face. */
+template <class T, int NMAX>
+__device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, const T* nA, const T* gA, T cA,
+                                    const T* gBW, Acc<T>& acc, int64_t& nf, const OutPtrs* o, int env, int ea,
+                                    int eb) {
+  T C[3];
+  T W2 = polygon_centroid<T, NMAX>(p, nA, C);
+  T pv[NMAX];
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) {
+    T x[3] = {p.x[i], p.y[i], p.z[i]};
+    pv[i] = cA + dot3(gA, x);
+  }
+  const T pC = cA + dot3(gA, C);
+  T nW[3], gAW[3];
+  rot(ic.pw.X_WA, nA, nW);
+  rot(ic.pw.X_WA, gA, gAW);
+  const int n = p.n;
+  if (o && o->poly_rec) {
+    unsigned long long slot = atomicAdd(o->poly_cnt, 1ull);
+    if ((int64_t)slot < o->poly_cap) {
+      double* P = o->poly_rec + slot * HCS_PREC;
+      P[0] = n;
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) {
+        T x[3] = {p.x[i], p.y[i], p.z[i]}, y[3];
+        xform(ic.pw.X_WA, x, y);
+        bool on = i < n;
+        P[1 + 3 * i] = on ? val(y[0]) : 0.0;
+        P[2 + 3 * i] = on ? val(y[1]) : 0.0;
+        P[3 + 3 * i] = on ? val(y[2]) : 0.0;
+        P[25 + i] = on ? val(pv[i]) : 0.0;
+      }
+#pragma unroll
+      for (int i = NMAX; i < 8; ++i) P[1 + 3 * i] = P[2 + 3 * i] = P[3 + 3 * i] = P[25 + i] = 0.0;
+      T y[3];
+      xform(ic.pw.X_WA, C, y);
+      P[33] = val(y[0]), P[34] = val(y[1]), P[35] = val(y[2]);
+      P[36] = val(pC);
+      P[37] = val(nW[0]), P[38] = val(nW[1]), P[39] = val(nW[2]);
+      o->poly_key[3 * slot] = env, o->poly_key[3 * slot + 1] = ea, o->poly_key[3 * slot + 2] = eb;
+      if (o->poly_tan) {
+        double* Q = o->poly_tan + slot * HCS_PREC;
+        Q[0] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) {
+          T x[3] = {p.x[i], p.y[i], p.z[i]}, y2[3];
+          xform(ic.pw.X_WA, x, y2);
+          bool on = i < n;
+          Q[1 + 3 * i] = on ? tangent(y2[0]) : 0.0;
+          Q[2 + 3 * i] = on ? tangent(y2[1]) : 0.0;
+          Q[3 + 3 * i] = on ? tangent(y2[2]) : 0.0;
+          Q[25 + i] = on ? tangent(pv[i]) : 0.0;
+        }
+#pragma unroll
+        for (int i = NMAX; i < 8; ++i) Q[1 + 3 * i] = Q[2 + 3 * i] = Q[3 + 3 * i] = Q[25 + i] = 0.0;
+        Q[33] = tangent(y[0]), Q[34] = tangent(y[1]), Q[35] = tangent(y[2]);
+        Q[36] = tangent(pC);
+        Q[37] = tangent(nW[0]), Q[38] = tangent(nW[1]), Q[39] = tangent(nW[2]);
+      }
+    }
+  }
+  const double third = 1.0 / 3.0;
+  if (ic.repr == 0) {
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k) {
+      if (k < n) {
+        const int k1 = (k + 1 == n) ? 0 : k + 1;
+        T a[3] = {p.x[k], p.y[k], p.z[k]};
+        T b[3] = {pick<T, NMAX>(p.x, k1), pick<T, NMAX>(p.y, k1), pick<T, NMAX>(p.z, k1)};
+        T pb = pick<T, NMAX>(pv, k1);
+        T e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        T e2[3] = {C[0] - a[0], C[1] - a[1], C[2] - a[2]};
+        T cr[3];
+        cross3(e1, e2, cr);
+        T area = 0.5 * dsqrt(dot3(cr, cr));
+        T cS[3] = {((a[0] + b[0]) + C[0]) / 3.0, ((a[1] + b[1]) + C[1]) / 3.0, ((a[2] + b[2]) + C[2]) / 3.0};
+        T p0 = (third * pv[k] + third * pb) + third * pC;
+        T cW[3];
+        xform(ic.pw.X_WA, cS, cW);
+        double rec[HCS_FREC];
+        bool want = o && o->face_rec;
+        if (eval_face(ic.pw, area, cW, p0, nW, gAW, gBW, acc, want ? rec : nullptr)) {
+          ++nf;
+          if (want) {
+            unsigned long long slot = atomicAdd(o->face_cnt, 1ull);
+            if ((int64_t)slot < o->face_cap) {
+#pragma unroll
+              for (int q = 0; q < HCS_FREC; ++q) o->face_rec[slot * HCS_FREC + q] = rec[q];
+              o->face_key[4 * slot] = env, o->face_key[4 * slot + 1] = ea, o->face_key[4 * slot + 2] = eb,
+                                o->face_key[4 * slot + 3] = k;
+            }
+          }
+        }
+      }
+    }
+  } else {
+    T area = 0.5 * W2;
+    T cW[3];
+    xform(ic.pw.X_WA, C, cW);
+    double rec[HCS_FREC];
+    bool want = o && o->face_rec;
+    if (eval_face(ic.pw, area, cW, pC, nW, gAW, gBW, acc, want ? rec : nullptr)) {
+      ++nf;
+      if (want) {
+        unsigned long long slot = atomicAdd(o->face_cnt, 1ull);
+        if ((int64_t)slot < o->face_cap) {
+#pragma unroll
+          for (int q = 0; q < HCS_FREC; ++q) o->face_rec[slot * HCS_FREC + q] = rec[q];
+          o->face_key[4 * slot] = env, o->face_key[4 * slot + 1] = ea, o->face_key[4 * slot + 2] = eb,
+                            o->face_key[4 * slot + 3] = 0;
+        }
+      }
+    }
+  }
+}
+
+/* One candidate (element a of A, element b of B). */
+template <class T>
+__device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic, int env, int a, int b, Acc<T>& acc,
+                                         int64_t& np, int64_t& nf, const OutPtrs* o) {
+  const double* r = E.recA + (size_t)HCS_TREC * a;
+  double rr[HCS_TREC];
+#pragma unroll
+  for (int i = 0; i < 23; ++i) rr[i] = r[i];
+  T gA[3] = {ic.escale * rr[16], ic.escale * rr[17], ic.escale * rr[18]};
+  T cA = ic.escale * rr[19];
+  if (E.kind != KIND_SOFT) {
+    /* soft-rigid: triangle b of the rigid mesh moved into A's frame */
+    const int32_t* f = E.FB + 3 * (size_t)b;
+    Poly<T> p;
+#pragma unroll
+    for (int i = 0; i < HCS_MAXV; ++i) p.x[i] = p.y[i] = p.z[i] = T(0.0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double* v = E.VB + 3 * (size_t)f[i];
+      T x[3] = {T(v[0]), T(v[1]), T(v[2])}, y[3];
+      xform(ic.X_AB, x, y);
+      p.x[i] = y[0], p.y[i] = y[1], p.z[i] = y[2];
+    }
+    p.n = 3;
+    const double* nr = E.nrmB + 3 * (size_t)b;
+    T nR[3] = {T(nr[0]), T(nr[1]), T(nr[2])}, nS[3];
+    rot(ic.X_AB, nR, nS);
+    /* Drake semantics (IsFaceNormalAlongPressureGradient) */
+    if (!(((rr[20] * val(nS[0]) + rr[21] * val(nS[1])) + rr[22] * val(nS[2])) > HCS_COS_ALPHA)) return;
+    clip_tri_tet(p, rr);
+    if (p.n < 3) return;
+    ++np;
+    T zero3[3] = {T(0.0), T(0.0), T(0.0)};
+    emit_polygon<T, 7>(ic, p, nS, gA, cA, zero3, acc, nf, o, env, a, b);
+  } else {
+    /* soft-soft: equal-pressure plane of tets a (A) and b (B) */
+    const double* rb = E.recB + (size_t)HCS_TREC * b;
+    double rbb[HCS_TREC];
+#pragma unroll
+    for (int i = 0; i < 23; ++i) rbb[i] = rb[i];
+    T gbB[3] = {T(rbb[16]), T(rbb[17]), T(rbb[18])}, ghB[3] = {T(rbb[20]), T(rbb[21]), T(rbb[22])};
+    T gb[3], gbh[3];
+    rot(ic.X_AB, gbB, gb);
+    rot(ic.X_AB, ghB, gbh);
+    T cb = T(rbb[19]) - dot3(gb, ic.X_AB + 9);
+    T nn[3] = {gA[0] - gb[0], gA[1] - gb[1], gA[2] - gb[2]};
+    T mag = dsqrt(dot3(nn, nn));
+    if (!(val(mag) > 0.0)) return;
+    T nh[3] = {nn[0] / mag, nn[1] / mag, nn[2] / mag};
+    T off = -((cb - cA) / mag);
+    if (!(((val(nh[0]) * rr[20] + val(nh[1]) * rr[21]) + val(nh[2]) * rr[22]) > HCS_COS_ALPHA)) return;
+    if (!(((-val(nh[0]) * val(gbh[0]) + -val(nh[1]) * val(gbh[1])) + -val(nh[2]) * val(gbh[2])) > HCS_COS_ALPHA))
+      return;
+    /* marching-tets slice of tet a; above the plane is s > 0 */
+    const int32_t* ta = E.TA + 4 * (size_t)a;
+    T vx[4], vy[4], vz[4], s[4];
+    int above = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double* v = E.VA + 3 * (size_t)ta[i];
+      vx[i] = T(v[0]), vy[i] = T(v[1]), vz[i] = T(v[2]);
+      T x[3] = {vx[i], vy[i], vz[i]};
+      s[i] = dot3(nh, x) + off;
+      above += val(s[i]) > 0;
+    }
+    if (above == 0 || above == 4) return;
+    int up[4] = {0, 0, 0, 0}, dn[4] = {0, 0, 0, 0}, nu = 0, nd = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (val(s[i]) > 0) {
+        up[0] = (nu == 0) ? i : up[0];
+        up[1] = (nu == 1) ? i : up[1];
+        up[2] = (nu == 2) ? i : up[2];
+        ++nu;
+      } else {
+        dn[0] = (nd == 0) ? i : dn[0];
+        dn[1] = (nd == 1) ? i : dn[1];
+        dn[2] = (nd == 2) ? i : dn[2];
+        ++nd;
+      }
+    }
+    int ei[4], ej[4], ne;
+    if (nu == 1 || nd == 1) {
+      const int lone = nu == 1 ? up[0] : dn[0];
+      int q = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i != lone) {
+          const int lo = lone < i ? lone : i, hi = lone < i ? i : lone;
+          ei[0] = q == 0 ? lo : ei[0], ej[0] = q == 0 ? hi : ej[0];
+          ei[1] = q == 1 ? lo : ei[1], ej[1] = q == 1 ? hi : ej[1];
+          ei[2] = q == 2 ? lo : ei[2], ej[2] = q == 2 ? hi : ej[2];
+          ++q;
+        }
+      }
+      ei[3] = ej[3] = 0;
+      ne = 3;
+    } else {
+      const int pa[4] = {up[0], up[0], up[1], up[1]}, pb[4] = {dn[0], dn[1], dn[1], dn[0]};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ei[e] = pa[e] < pb[e] ? pa[e] : pb[e], ej[e] = pa[e] < pb[e] ? pb[e] : pa[e];
+      ne = 4;
+    }
+    Poly<T> p;
+#pragma unroll
+    for (int i = 0; i < HCS_MAXV; ++i) p.x[i] = p.y[i] = p.z[i] = T(0.0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (e < ne) {
+        T si = pick<T, 4>(s, ei[e]), sj = pick<T, 4>(s, ej[e]);
+        T t = si / (si - sj);
+        T xi = pick<T, 4>(vx, ei[e]), yi = pick<T, 4>(vy, ei[e]), zi = pick<T, 4>(vz, ei[e]);
+        T xj = pick<T, 4>(vx, ej[e]), yj = pick<T, 4>(vy, ej[e]), zj = pick<T, 4>(vz, ej[e]);
+        p.x[e] = xi + t * (xj - xi);
+        p.y[e] = yi + t * (yj - yi);
+        p.z[e] = zi + t * (zj - zi);
+      }
+    }
+    p.n = ne;
+    /* orient counter-clockwise about nh */
+    T W = T(0.0);
+#pragma unroll
+    for (int i = 1; i + 1 < 4; ++i) {
+      if (i + 1 < ne) {
+        T e1[3] = {p.x[i] - p.x[0], p.y[i] - p.y[0], p.z[i] - p.z[0]};
+        T e2[3] = {p.x[i + 1] - p.x[0], p.y[i + 1] - p.y[0], p.z[i + 1] - p.z[0]};
+        T cr[3];
+        cross3(e1, e2, cr);
+        W = W + dot3(cr, nh);
+      }
+    }
+    if (val(W) < 0) {
+      if (ne == 3) {
+        T tx = p.x[1], ty = p.y[1], tz = p.z[1];
+        p.x[1] = p.x[2], p.y[1] = p.y[2], p.z[1] = p.z[2];
+        p.x[2] = tx, p.y[2] = ty, p.z[2] = tz;
+      } else {
+        T tx = p.x[1], ty = p.y[1], tz = p.z[1];
+        p.x[1] = p.x[3], p.y[1] = p.y[3], p.z[1] = p.z[3];
+        p.x[3] = tx, p.y[3] = ty, p.z[3] = tz;
+      }
+    }
+    /* tet b's half-spaces in A's frame */
+    T pn[12], po[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      T nb[3] = {T(rbb[3 * k]), T(rbb[3 * k + 1]), T(rbb[3 * k + 2])};
+      rot(ic.X_AB, nb, pn + 3 * k);
+      po[k] = T(rbb[12 + k]) - dot3(pn + 3 * k, ic.X_AB + 9);
+    }
+    clip_quad_tet(p, pn, po);
+    if (p.n < 3) return;
+    ++np;
+    T gBW[3];
+    rot(ic.pw.X_WB, gbB, gBW);
+    emit_polygon<T, 8>(ic, p, nh, gA, cA, gBW, acc, nf, o, env, a, b);
+  }
+}
+
+__device__ inline bool box_overlap(const double* a /*lo,hi*/, const double* lo, const double* hi) {
+  return a[0] <= hi[0] && lo[0] <= a[3] && a[1] <= hi[1] && lo[1] <= a[4] && a[2] <= hi[2] && lo[2] <= a[5];
+}
+
+__device__ inline unsigned lane_prefix(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+template <class T>
+__device__ inline T wave_sum(T v);
+template <>
+__device__ inline double wave_sum<double>(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  return v;
+}
+template <>
+__device__ inline Dual wave_sum<Dual>(Dual v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    double a = __shfl_xor(v.v, o, 64), b = __shfl_xor(v.d, o, 64);
+    v.v = v.v + a;
+    v.d = v.d + b;
+  }
+  return v;
+}
+__device__ inline int64_t wave_sum_i(int64_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+/* Shared-memory layout of one wave. */
+struct WaveLds {
+  int2 q[192];
+  int stk[HCS_STACK][64];
+  double tile[64 * 6];
+};
+
+/* Processes the top 64 (or all remaining) queued candidates. */
+template <class T>
+__device__ inline void drain(WaveLds& L, int& qn, bool all, const EnvDesc& E, const ItemCtx<T>& ic, int env,
+                             Acc<T>& acc, int64_t& nc, int64_t& np, int64_t& nf, const OutPtrs* o,
+                             const QueryArgs& A, int lane) {
+  while (qn >= 64 || (all && qn > 0)) {
+    __syncthreads();
+    const int take = qn >= 64 ? 64 : qn;
+    const int base = qn - take;
+    if (lane < take) {
+      int2 c = L.q[base + lane];
+      ++nc;
+      if (A.cand_key) {
+        unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
+        if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
+      } else {
+        process_candidate<T>(E, ic, env, c.x, c.y, acc, np, nf, o);
+      }
+    }
+    qn = base;
+    __syncthreads();
+  }
+}
+
+/* Body of the query kernel for one item. */
+template <class T>
+__device__ inline void run_item(WaveLds& L, const QueryArgs& A, int it, int dir, const double* dstar_dE,
+                                const double* invE, Partial* out, const OutPtrs* o) {
+  const int lane = threadIdx.x & 63;
+  const Item item = A.items[it];
+  const int env = item.env;
+  const EnvDesc E = A.envs[env];
+  const double* st = A.state + (size_t)HCS_STATE * env;
+  ItemCtx<T> ic;
+  make_item_ctx<T>(E, st, dir, dstar_dE ? dstar_dE[env] : 0.0, invE ? invE[env] : 0.0, ic);
+  double X[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) X[i] = val(ic.X_AB[i]);
+  Acc<T> acc;
+  acc.zero();
+  int64_t nc = 0, np = 0, nf = 0;
+  int qn = 0;
+  if (E.kind == KIND_TET) {
+    /* queries are tets of A; the rigid triangles come through LDS tiles */
+    const int t = item.first + lane;
+    const bool valid = lane < item.count;
+    double bx[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) bx[c] = valid ? E.boxA[6 * (size_t)t + c] : 0.0;
+    for (int t0 = 0; t0 < E.nB; t0 += 64) {
+      const int nt = (E.nB - t0) < 64 ? (E.nB - t0) : 64;
+      __syncthreads();
+      if (lane < nt) {
+        const int32_t* f = E.FB + 3 * (size_t)(t0 + lane);
+        double lo[3], hi[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double y[3];
+          const double* v = E.VB + 3 * (size_t)f[i];
+          xform(X, v, y);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
+            hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) L.tile[6 * lane + c] = lo[c], L.tile[6 * lane + 3 + c] = hi[c];
+      }
+      __syncthreads();
+      for (int j = 0; j < nt; ++j) {
+        const double* tb = &L.tile[6 * j];
+        const bool hit = valid && box_overlap(bx, tb, tb + 3);
+        const unsigned long long m = __ballot(hit);
+        if (hit) L.q[qn + lane_prefix(m)] = make_int2(t, t0 + j);
+        qn += __popcll(m);
+        drain<T>(L, qn, false, E, ic, env, acc, nc, np, nf, o, A, lane);
+      }
+      drain<T>(L, qn, true, E, ic, env, acc, nc, np, nf, o, A, lane);
+    }
+  } else {
+    for (int sub = 0; sub < A.queries_per_lane; ++sub) {
+      const int qi = sub * 64 + lane;
+      const bool valid = qi < item.count;
+      const int b = item.first + qi;
+      double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+      if (valid) {
+        const int nvq = E.kind == KIND_SOFT ? 4 : 3;
+        const int32_t* f = E.FB + (size_t)nvq * b;
+        for (int i = 0; i < nvq; ++i) {
+          double y[3];
+          xform(X, E.VB + 3 * (size_t)f[i], y);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
+            hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+          }
+        }
+      }
+      int sp = 0;
+      if (E.ntA == 1) {
+        const bool hit = valid && box_overlap(E.boxA, lo, hi);
+        const unsigned long long m = __ballot(hit);
+        if (hit) L.q[qn + lane_prefix(m)] = make_int2(0, b);
+        qn += __popcll(m);
+        drain<T>(L, qn, false, E, ic, env, acc, nc, np, nf, o, A, lane);
+      } else if (valid) {
+        L.stk[0][lane] = 0;
+        sp = 1;
+      }
+      /* A tree of ntA leaves has ntA-1 internal nodes, each popped at most
+       * once per query; the cap only guards against a malformed tree. */
+      int visits = 0;
+      while (__any(sp > 0)) {
+        int c0 = -1, c1 = -1;
+        if (sp > 0 && ++visits > E.ntA) {
+          atomicOr(A.error, 2);
+          sp = 0;
+        }
+        if (sp > 0) {
+          const int nd = L.stk[--sp][lane];
+          const Node* N = E.nodesA + nd;
+          double bb[12];
+#pragma unroll
+          for (int i = 0; i < 12; ++i) bb[i] = N->b[i];
+          const int ch0 = N->c[0], ch1 = N->c[1];
+          const bool o0 = box_overlap(bb, lo, hi), o1 = box_overlap(bb + 6, lo, hi);
+          if (o1) {
+            if (ch1 < 0) c1 = ~ch1;
+            else if (sp < HCS_STACK) L.stk[sp++][lane] = ch1;
+            else atomicOr(A.error, 1);
+          }
+          if (o0) {
+            if (ch0 < 0) c0 = ~ch0;
+            else if (sp < HCS_STACK) L.stk[sp++][lane] = ch0;
+            else atomicOr(A.error, 1);
+          }
+        }
+        const unsigned long long m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
+        if (c0 >= 0) L.q[qn + lane_prefix(m0)] = make_int2(c0, b);
+        if (c1 >= 0) L.q[qn + __popcll(m0) + lane_prefix(m1)] = make_int2(c1, b);
+        qn += __popcll(m0) + __popcll(m1);
+        drain<T>(L, qn, false, E, ic, env, acc, nc, np, nf, o, A, lane);
+      }
+    }
+    drain<T>(L, qn, true, E, ic, env, acc, nc, np, nf, o, A, lane);
+  }
+  /* fixed-order wave reduction */
+  Partial r;
+  T F[3], tA[3], tB[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    F[c] = wave_sum<T>(acc.F[c]);
+    tA[c] = wave_sum<T>(acc.tA[c]);
+    tB[c] = wave_sum<T>(acc.tB[c]);
+  }
+  const int64_t snc = wave_sum_i(nc), snp = wave_sum_i(np), snf = wave_sum_i(nf);
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      r.F[c] = dir < 0 ? val(F[c]) : tangent(F[c]);
+      r.tA[c] = dir < 0 ? val(tA[c]) : tangent(tA[c]);
+      r.tB[c] = dir < 0 ? val(tB[c]) : tangent(tB[c]);
+    }
+    r.nc = snc, r.np = snp, r.nf = snf;
+    *out = r;
+  }
+}
+
+}  // namespace hcs

@@ -1,0 +1,315 @@
+/*
+ * hcs_taxels.hip -- flat tactile sensor sampling on the GPU, FP64.
+ *
+ * Restates FlatTactileSensor::bvh_update
+ * (mujoco_contact_surface_sensors/src/flat_tactile_sensor.cpp:262-402): per
+ * taxel (x, y) and sub-sample (i, j) a ray from
+ *   R (-xs + x res + (i+1/2) res/S, -ys + y res + (j+1/2) res/S, 1.5 zs) + t
+ * offset by 1e-8 along the sensor normal n, direction -n, nearest hit over
+ * the kTriangle fans of every contact polygon of the pairs touching the
+ * sensor geometry (Moeller-Trumbore as in
+ * mujoco_contact_surface_sensors/src/bvh.cpp:49-74, |det| < 1e-10 rejected),
+ * accepted when 0 < t < 1.5 zs; value Evaluate(tri, (1-u-v, u, v)) / S^2
+ * times the window weight, summed per taxel.
+ *
+ * Instead of the reference's per-update float32 BVH + TLAS over the contact
+ * triangles, all rays are parallel, so the polygons are binned once into the
+ * taxel grid by their footprint in the sensor plane; one workgroup per taxel
+ * then tests only its bin.  Ties in t are broken by (polygon key, fan index),
+ * the canonical polygon order.
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hcs_device.h"
+#include "hcs_internal.h"
+
+#pragma clang fp contract(off)
+
+using namespace hcs;
+
+namespace hcs {
+
+__device__ inline void footprint(const double* P, const TaxelGridDev& g, int& x0, int& x1, int& y0, int& y1) {
+  const int n = (int)P[0];
+  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+  for (int i = 0; i < n; ++i) {
+    const double* v = P + 1 + 3 * i;
+    double d[3] = {v[0] - g.X[9], v[1] - g.X[10], v[2] - g.X[11]};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      double l = (g.X[c] * d[0] + g.X[3 + c] * d[1]) + g.X[6 + c] * d[2];
+      lo[c] = fmin(lo[c], l), hi[c] = fmax(hi[c], l);
+    }
+  }
+  const double m = 1e-9;
+  x0 = (int)floor((lo[0] - m + g.xs) / g.res);
+  x1 = (int)floor((hi[0] + m + g.xs) / g.res);
+  y0 = (int)floor((lo[1] - m + g.ys) / g.res);
+  y1 = (int)floor((hi[1] + m + g.ys) / g.res);
+  x0 = max(x0, 0), y0 = max(y0, 0), x1 = min(x1, g.cx - 1), y1 = min(y1, g.cy - 1);
+}
+
+__global__ void k_tx_bin(const double* __restrict__ poly, const int64_t* __restrict__ key, int64_t np,
+                         const unsigned char* __restrict__ env_mask, TaxelGridDev g, unsigned* __restrict__ cnt,
+                         const unsigned* __restrict__ off, unsigned* __restrict__ cursor, int* __restrict__ list) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  if (!env_mask[key[3 * q]]) return;
+  const double* P = poly + q * HCS_PREC;
+  int x0, x1, y0, y1;
+  footprint(P, g, x0, x1, y0, y1);
+  for (int y = y0; y <= y1; ++y)
+    for (int x = x0; x <= x1; ++x) {
+      const int cell = x + g.cx * y;
+      if (!off) {
+        atomicAdd(cnt + cell, 1u);
+      } else {
+        const unsigned slot = atomicAdd(cursor + cell, 1u);
+        list[off[cell] + slot] = (int)q;
+      }
+    }
+}
+
+__device__ inline int64_t pkey(const int64_t* key, int q) {
+  return (key[3 * q] << 42) | (key[3 * q + 1] << 21) | key[3 * q + 2];
+}
+
+/* One workgroup per taxel; samples strided over the threads, each thread
+ * sums its samples in index order, then a fixed tree reduction.  T = Dual
+ * carries one tangent direction: polygon tangents in ptan, sensor pose
+ * tangent in Xd (the hit triangle is chosen on primal values). */
+template <class T>
+__device__ inline T ld(const double* v, const double* d, size_t i);
+template <>
+__device__ inline double ld<double>(const double* v, const double*, size_t i) {
+  return v[i];
+}
+template <>
+__device__ inline Dual ld<Dual>(const double* v, const double* d, size_t i) {
+  return Dual(v[i], d[i]);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tx_sample(const double* __restrict__ poly, const double* __restrict__ ptan,
+                                                   const int64_t* __restrict__ key, TaxelGridDev g,
+                                                   const double* __restrict__ Xd, const unsigned* __restrict__ off,
+                                                   const unsigned* __restrict__ cnt, const int* __restrict__ list,
+                                                   double* __restrict__ out) {
+  const int cell = blockIdx.x;
+  const int x = cell % g.cx, y = cell / g.cx;
+  const int S = g.S;
+  const double rSR = g.res / S;
+  const double rmean = 1. / (S * S);
+  const double di_factor = g.res / S;
+  const double sub_halfwidth = di_factor / 2.0 - g.res / 2.0;
+  const double max_dist = 1.41421356237 * g.res / 2.0;
+  T X[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) X[i] = ld<T>(g.X, Xd, i);
+  const T nrm[3] = {X[2], X[5], X[8]};
+  double rsigma_squared = 0, rtukey = 0;
+  if (g.window == 1) rsigma_squared = 0.5 / (g.sigma * g.sigma);
+  if (g.window == 2) rtukey = 1. / g.sigma * S * S;
+  const double PI = 3.14159265358979323846;
+  const unsigned base = off[cell], n = cnt[cell];
+  T acc = T(0.0);
+  for (int s = threadIdx.x; s < S * S; s += blockDim.x) {
+    const int i = s / S, j = s % S;
+    T loc[3] = {T(-g.xs + x * g.res + i * rSR + 0.5 * rSR), T(-g.ys + y * g.res + j * rSR + 0.5 * rSR),
+                T(1.5 * g.zs)};
+    T pt[3];
+    xform(X, loc, pt);
+    T O[3], D[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) O[c] = pt[c] + nrm[c] * 1e-8, D[c] = -nrm[c];
+    double best_t = 1e30;
+    int64_t bkey = INT64_MAX;
+    int bk = 0, bq = -1;
+    for (unsigned l = 0; l < n; ++l) {
+      const int q = list[base + l];
+      const double* P = poly + (size_t)q * HCS_PREC;
+      const int nv = (int)P[0];
+      const int64_t kq = pkey(key, q);
+      double Ov[3] = {val(O[0]), val(O[1]), val(O[2])}, Dv[3] = {val(D[0]), val(D[1]), val(D[2])};
+      for (int k = 0; k < nv; ++k) {
+        const int k1 = (k + 1 == nv) ? 0 : k + 1;
+        const double* v0 = P + 1 + 3 * k;
+        const double* v1 = P + 1 + 3 * k1;
+        const double* v2 = P + 33;
+        double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+        double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+        double h[3];
+        cross3(Dv, e2, h);
+        const double a = dot3(e1, h);
+        if (fabs(a) < 1e-10) continue;
+        const double f = 1.0 / a;
+        double sv[3] = {Ov[0] - v0[0], Ov[1] - v0[1], Ov[2] - v0[2]};
+        const double u = f * dot3(sv, h);
+        if (u < 0.0 || u > 1.0) continue;
+        double qv[3];
+        cross3(sv, e1, qv);
+        const double v = f * dot3(Dv, qv);
+        if (v < 0.0 || u + v > 1.0) continue;
+        const double t = f * dot3(e2, qv);
+        if (!(t > 0.0)) continue;
+        const bool better = t < best_t || (t == best_t && (kq < bkey || (kq == bkey && k < bk)));
+        if (better) best_t = t, bkey = kq, bk = k, bq = q;
+      }
+    }
+    if (bq >= 0 && best_t < 1.5 * g.zs && best_t > 0.0) {
+      /* Moeller-Trumbore on the chosen triangle, in T arithmetic */
+      const size_t pb = (size_t)bq * HCS_PREC;
+      const int nv = (int)poly[pb];
+      const int k1 = (bk + 1 == nv) ? 0 : bk + 1;
+      T v0[3], v1[3], v2[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        v0[c] = ld<T>(poly, ptan, pb + 1 + 3 * bk + c);
+        v1[c] = ld<T>(poly, ptan, pb + 1 + 3 * k1 + c);
+        v2[c] = ld<T>(poly, ptan, pb + 33 + c);
+      }
+      T e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+      T e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+      T h[3];
+      cross3(D, e2, h);
+      const T f = T(1.0) / dot3(e1, h);
+      T sv[3] = {O[0] - v0[0], O[1] - v0[1], O[2] - v0[2]};
+      const T u = f * dot3(sv, h);
+      T qv[3];
+      cross3(sv, e1, qv);
+      const T v = f * dot3(D, qv);
+      const T pa = ld<T>(poly, ptan, pb + 25 + bk), pbb = ld<T>(poly, ptan, pb + 25 + k1),
+              pc = ld<T>(poly, ptan, pb + 36);
+      const T b0 = T(1.0) - u - v;
+      const T raw = ((b0 * pa + u * pbb) + v * pc) * rmean;
+      double weight = 1.0;
+      if (g.window == 1) {
+        const double dist = hypot(di_factor * i + sub_halfwidth, di_factor * j + sub_halfwidth) / max_dist;
+        weight = exp(-(dist * dist) * rsigma_squared);
+      } else if (g.window == 2) {
+        const int ni = S / 2 - abs(S / 2 - i), nj = S / 2 - abs(S / 2 - j);
+        if (ni <= g.sigma * S / 2) weight *= 0.5 * (1. - cos(2. * PI * ni * rtukey));
+        if (nj <= g.sigma * S / 2) weight *= 0.5 * (1. - cos(2. * PI * nj * rtukey));
+      } else if (g.window == 3) {
+        const double inv = 1 - hypot(di_factor * i + sub_halfwidth, di_factor * j + sub_halfwidth) / max_dist;
+        weight = inv * inv;
+      }
+      acc = acc + weight * raw;
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = ptan ? tangent(acc) : val(acc);
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[x + g.cx * y] = red[0];
+}
+
+/* Bins polygons of the masked pairs into the taxel grid and samples it.
+ * ptan/Xd select the tangent pass (nullptr: values). */
+int run_taxels(Ctx* c, const TaxelGridDev& g, const unsigned char* d_mask, const double* poly, const double* ptan,
+               const int64_t* key, int64_t np, const double* Xd, double* d_out) {
+  const int ncell = g.cx * g.cy;
+  int rc;
+  if ((rc = ensure(c, c->d_tx_cnt, sizeof(unsigned) * 2 * ncell)) ||
+      (rc = ensure(c, c->d_tx_off, sizeof(unsigned) * (ncell + 1))))
+    return rc;
+  unsigned* cnt = (unsigned*)c->d_tx_cnt.ptr;
+  unsigned* cursor = cnt + ncell;
+  unsigned* off = (unsigned*)c->d_tx_off.ptr;
+  rc = hip_check(c, hipMemsetAsync(cnt, 0, sizeof(unsigned) * 2 * ncell, c->stream), "memset");
+  if (rc) return rc;
+  const int64_t nb = (np + 255) / 256;
+  if (np > 0)
+    hipLaunchKernelGGL(k_tx_bin, dim3((unsigned)nb), dim3(256), 0, c->stream, poly, key, np, d_mask, g, cnt,
+                       (const unsigned*)nullptr, (unsigned*)nullptr, (int*)nullptr);
+  std::vector<unsigned> hc(ncell), ho(ncell + 1);
+  rc = hip_check(c, hipMemcpyAsync(hc.data(), cnt, sizeof(unsigned) * ncell, hipMemcpyDeviceToHost, c->stream),
+                 "download counts");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  ho[0] = 0;
+  for (int i = 0; i < ncell; ++i) ho[i + 1] = ho[i] + hc[i];
+  if ((rc = ensure(c, c->d_tx_list, sizeof(int) * std::max<unsigned>(1, ho[ncell])))) return rc;
+  rc = hip_check(c, hipMemcpyAsync(off, ho.data(), sizeof(unsigned) * (ncell + 1), hipMemcpyHostToDevice, c->stream),
+                 "upload offsets");
+  if (rc) return rc;
+  if (np > 0)
+    hipLaunchKernelGGL(k_tx_bin, dim3((unsigned)nb), dim3(256), 0, c->stream, poly, key, np, d_mask, g, cnt,
+                       (const unsigned*)off, cursor, (int*)c->d_tx_list.ptr);
+  if (ptan)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tx_sample<Dual>), dim3(ncell), dim3(256), 0, c->stream, poly, ptan, key, g,
+                       Xd, (const unsigned*)off, (const unsigned*)cnt, (const int*)c->d_tx_list.ptr, d_out);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tx_sample<double>), dim3(ncell), dim3(256), 0, c->stream, poly,
+                       (const double*)nullptr, key, g, (const double*)nullptr, (const unsigned*)off,
+                       (const unsigned*)cnt, (const int*)c->d_tx_list.ptr, d_out);
+  return hip_check(c, hipGetLastError(), "taxel kernels");
+}
+
+int taxel_grid_dev(const hcs_taxel_grid& G, TaxelGridDev& g) {
+  int32_t cx, cy;
+  if (hcs_taxel_dims(&G, &cx, &cy) != HCS_OK || G.sampling_resolution <= 0 || G.window < 0 || G.window > 3)
+    return HCS_ERR_ARG;
+  std::memcpy(g.X, G.X_W_sensor, sizeof g.X);
+  g.xs = G.half_size[0], g.ys = G.half_size[1], g.zs = G.half_size[2], g.res = G.resolution, g.sigma = G.sigma;
+  g.S = G.sampling_resolution, g.window = G.window, g.cx = cx, g.cy = cy;
+  return HCS_OK;
+}
+
+}  // namespace hcs
+
+extern "C" {
+
+int hcs_taxel_dims(const hcs_taxel_grid* g, int32_t* cx, int32_t* cy) {
+  if (!g || !cx || !cy || !(g->resolution > 0)) return HCS_ERR_ARG;
+  /* ::floorl(2 * xs / resolution + 0.1), flat_tactile_sensor.cpp:196-197 */
+  *cx = (int32_t)std::floor(2 * (long double)g->half_size[0] / g->resolution + 0.1L);
+  *cy = (int32_t)std::floor(2 * (long double)g->half_size[1] / g->resolution + 0.1L);
+  if (*cx <= 0 || *cy <= 0) return HCS_ERR_ARG;
+  return HCS_OK;
+}
+
+int hcs_taxels(hcs_ctx* h, int32_t nsensors, const hcs_taxel_grid* grids, double* out) {
+  if (!h || nsensors < 0 || (nsensors > 0 && (!grids || !out))) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!c->last_valid) return set_err(c, HCS_ERR_STATE, "hcs_taxels needs a contact step run with keep_polygons = 1");
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  const int npairs = (int)c->last_pairs.size();
+  size_t off_out = 0;
+  for (int s = 0; s < nsensors; ++s) {
+    const hcs_taxel_grid& G = grids[s];
+    TaxelGridDev g;
+    if (taxel_grid_dev(G, g) != HCS_OK) return set_err(c, HCS_ERR_ARG, "invalid taxel grid");
+    const int ncell = g.cx * g.cy;
+    std::vector<unsigned char> mask(std::max(1, npairs), 0);
+    for (int e = 0; e < npairs; ++e)
+      mask[e] = (c->last_geoms[2 * e] == G.geom || c->last_geoms[2 * e + 1] == G.geom) ? 1 : 0;
+    if ((rc = ensure(c, c->d_tx_grid, mask.size())) || (rc = ensure(c, c->d_tx_out, sizeof(double) * ncell)))
+      return rc;
+    rc = hip_check(c, hipMemcpyAsync(c->d_tx_grid.ptr, mask.data(), mask.size(), hipMemcpyHostToDevice, c->stream),
+                   "upload mask");
+    if (!rc)
+      rc = run_taxels(c, g, (const unsigned char*)c->d_tx_grid.ptr, (const double*)c->d_poly_rec.ptr, nullptr,
+                      (const int64_t*)c->d_poly_key.ptr, c->last_npoly, nullptr, (double*)c->d_tx_out.ptr);
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(out + off_out, c->d_tx_out.ptr, sizeof(double) * ncell, hipMemcpyDeviceToHost,
+                                       c->stream),
+                     "download taxels");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
+    off_out += ncell;
+  }
+  return HCS_OK;
+}
+
+}  // extern "C"

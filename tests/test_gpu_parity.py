@@ -1,0 +1,365 @@
+"""GPU parity: libhcs (HIP, gfx950) against the CPU oracle on the same seeded
+inputs.  Bar: candidate sets bit-exact; areas, pressures, wrenches and
+taxels within 1e-10 relative (wrench blocks norm-relative); derivatives
+within 1e-6 of central finite differences (step 1e-7)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mujoco_contact_surfaces_amd import scenes as S
+from meshes import pose
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def add_scene(engine, sc):
+    s, r = sc["soft"], sc.get("rigid")
+    ha = engine.add_soft_mesh(s["V"], s["T"], s["p"], s["props"], s["geom_id"])
+    if r is not None:
+        hb = engine.add_rigid_mesh(r["V"], r["F"], r["props"], r["geom_id"])
+    else:
+        b = sc["soft_b"]
+        hb = engine.add_soft_mesh(b["V"], b["T"], b["p"], b["props"], b["geom_id"])
+    return ha, hb
+
+
+def pair_of(sc, ha, hb, X_WB=None):
+    return (ha, hb, sc["X_WA"], sc["X_WB"] if X_WB is None else X_WB, sc["V_WA"], sc["V_WB"])
+
+
+def oracle_soft_rigid(sc, representation=0, apply=True, X_WB=None, brute=False):
+    s, r = sc["soft"], sc["rigid"]
+    return O.soft_rigid(s["V"], s["T"], s["p"], r["V"], r["F"], sc["X_WA"], sc["X_WB"] if X_WB is None else X_WB,
+                        sc["V_WA"], sc["V_WB"], s["props"]["hydroelastic_modulus"], s["props"]["dissipation"],
+                        s["props"]["dynamic_friction"], r["props"]["dynamic_friction"],
+                        representation=representation, apply=apply, soft_is_M=s["geom_id"] < r["geom_id"],
+                        brute=brute)
+
+
+def assert_wrench_close(got, ref, rtol=RTOL):
+    got, ref = np.asarray(got), np.asarray(ref)
+    scale = np.linalg.norm(ref[:3]) + 1e-300
+    for b in range(4):
+        g, r = got[3 * b:3 * b + 3], ref[3 * b:3 * b + 3]
+        err = np.linalg.norm(g - r)
+        assert err <= rtol * np.linalg.norm(r) + 1e-14 * scale, (b, g, r, err)
+
+
+def match_faces(rec, key, ref_rec, ref_key, rtol=RTOL):
+    """GPU faces (keys pair, a, b, k) against oracle faces (keys a, b, k)."""
+    assert rec.shape[0] == ref_rec.shape[0]
+    o1 = np.lexsort((key[:, 3], key[:, 2], key[:, 1]))
+    o2 = np.lexsort((ref_key[:, 2], ref_key[:, 1], ref_key[:, 0]))
+    np.testing.assert_array_equal(key[o1][:, 1:], ref_key[o2])
+    a, b = rec[o1], ref_rec[o2]
+    np.testing.assert_allclose(a, b, rtol=rtol, atol=1e-300)
+    return np.abs(a - b).max()
+
+
+# ------------------------------------------------------------------ records
+
+
+def test_tet_records_bit_exact(engine):
+    say("tet records, sphere_box soft mesh")
+    sc = S.sphere_box(0)
+    ha, hb = add_scene(engine, sc)
+    s = sc["soft"]
+    ref = O.tet_records(s["V"], s["T"], s["p"])[:, :23]
+    got = engine.tet_records(ha)[:, :23]
+    np.testing.assert_array_equal(got, ref)
+    assert engine.mesh_info(ha)["bvh_height"] < 62
+
+
+# --------------------------------------------------------------- candidates
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_candidates_bit_exact_sphere_box(engine, seed):
+    say("candidates sphere_box seed", seed)
+    sc = S.sphere_box(seed)
+    ha, hb = add_scene(engine, sc)
+    s, r = sc["soft"], sc["rigid"]
+    ref = O.candidates(s["V"], s["T"], r["V"], r["F"], sc["X_WA"], sc["X_WB"], brute=True)
+    got = engine.candidates(pair_of(sc, ha, hb))
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_candidates_bit_exact_tactile_pad(engine):
+    say("candidates tactile_pad")
+    sc = S.tactile_pad(0)
+    ha, hb = add_scene(engine, sc)
+    s, r = sc["soft"], sc["rigid"]
+    ref = O.candidates(s["V"], s["T"], r["V"], r["F"], sc["X_WA"], sc["X_WB"])
+    got = engine.candidates(pair_of(sc, ha, hb))
+    assert got.size > 10000
+    np.testing.assert_array_equal(got, ref)
+
+
+# ---------------------------------------------------------- contact parity
+
+
+@pytest.mark.parametrize("representation", [0, 1])
+def test_contact_sphere_box(engine, representation):
+    say("contact sphere_box repr", representation)
+    sc = S.sphere_box(0)
+    ha, hb = add_scene(engine, sc)
+    ref = oracle_soft_rigid(sc, representation)
+    W, cnt, (rec, key) = engine.contact_step([pair_of(sc, ha, hb)], representation, face_capacity=4096)
+    np.testing.assert_array_equal(cnt[0], ref["counts"])
+    assert_wrench_close(W[0], ref["wrench"])
+    match_faces(rec, key, ref["faces"], ref["face_keys"])
+
+
+def test_contact_and_taxels_tactile_pad(engine):
+    say("contact + taxels tactile_pad")
+    sc = S.tactile_pad(0)
+    ha, hb = add_scene(engine, sc)
+    ref = oracle_soft_rigid(sc)
+    W, cnt, (rec, key) = engine.contact_step([pair_of(sc, ha, hb)], keep_polygons=True, face_capacity=1 << 16)
+    np.testing.assert_array_equal(cnt[0], ref["counts"])
+    assert_wrench_close(W[0], ref["wrench"])
+    match_faces(rec, key, ref["faces"], ref["face_keys"])
+    g = dict(sc["grid"])
+    for window, sigma in ((0, 0.0), (1, 0.3), (3, 0.0)):
+        say("taxels window", window)
+        g.update(window=window, sigma=sigma)
+        tx = engine.taxels([dict(geom=hb, X_W_sensor=sc["X_WB"], **g)])[0]
+        txr = O.taxels(ref["polys"], sc["X_WB"], g["half_size"], g["resolution"], g["sampling_resolution"], window,
+                       sigma)
+        assert tx.shape == (16, 16)
+        assert np.abs(txr).max() > 0
+        np.testing.assert_allclose(tx, txr, rtol=RTOL, atol=RTOL * np.abs(txr).max())
+
+
+def test_swapped_geometry_ids_give_the_same_wrench(engine):
+    say("M/N swap")
+    sc = S.sphere_box(2)
+    ha, hb = add_scene(engine, sc)
+    sc2 = S.sphere_box(2)
+    sc2["soft"]["geom_id"], sc2["rigid"]["geom_id"] = 7, 3
+    ha2, hb2 = add_scene(engine, sc2)
+    W1, c1, (r1, k1) = engine.contact_step([pair_of(sc, ha, hb)], face_capacity=4096)
+    W2, c2, (r2, k2) = engine.contact_step([pair_of(sc2, ha2, hb2)], face_capacity=4096)
+    np.testing.assert_array_equal(W1, W2)
+    ref = oracle_soft_rigid(sc2)
+    match_faces(r2, k2, ref["faces"], ref["face_keys"])
+
+
+def test_apply_forces_off(engine):
+    say("apply_forces off")
+    sc = S.sphere_box(1)
+    ha, hb = add_scene(engine, sc)
+    W, cnt = engine.contact_step([pair_of(sc, ha, hb)], apply_forces=False)
+    ref = oracle_soft_rigid(sc, apply=False)
+    assert np.all(W == 0)
+    np.testing.assert_array_equal(cnt[0], ref["counts"])
+
+
+def test_separated_bodies_give_nothing(engine):
+    say("no contact")
+    sc = S.sphere_box(0)
+    ha, hb = add_scene(engine, sc)
+    X = sc["X_WA"].copy()
+    X[11] += 1.0
+    W, cnt = engine.contact_step([(ha, hb, X, sc["X_WB"], sc["V_WA"], sc["V_WB"])])
+    assert np.all(W == 0) and np.all(cnt == 0)
+
+
+def test_run_to_run_bit_identical(engine):
+    say("determinism")
+    sc = S.tactile_pad(1, L=3, K=3, m=32)
+    ha, hb = add_scene(engine, sc)
+    a = engine.contact_step([pair_of(sc, ha, hb)])
+    b = engine.contact_step([pair_of(sc, ha, hb)])
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_many_pairs_in_one_call_match_single_calls(engine):
+    say("multi-pair call")
+    scs = [S.sphere_box(k) for k in range(4)]
+    pairs = []
+    for sc in scs:
+        ha, hb = add_scene(engine, sc)
+        pairs.append(pair_of(sc, ha, hb))
+    W, cnt = engine.contact_step(pairs)
+    for i, p in enumerate(pairs):
+        Wi, ci = engine.contact_step([p])
+        np.testing.assert_array_equal(W[i], Wi[0])
+        np.testing.assert_array_equal(cnt[i], ci[0])
+
+
+# ------------------------------------------------------------------ batched
+
+
+def _small_batch(engine, n=24, seed=5):
+    scs = [S.batched_env(seed, e, nt_range=(2e3, 1.5e4), ntri_range=(12, 3e3)) for e in range(n)]
+    handles = [add_scene(engine, sc) for sc in scs]
+    state = np.array([np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]]) for sc in scs])
+    return scs, handles, state
+
+
+def test_batched_matches_plain_and_oracle(engine):
+    say("batched small")
+    scs, handles, state = _small_batch(engine)
+    b = engine.batch(np.array(handles, dtype=np.int32))
+    b.set_state(state)
+    b.step()
+    W, cnt = b.get()
+    for i, (sc, (ha, hb)) in enumerate(zip(scs, handles)):
+        Wp, cp = engine.contact_step([pair_of(sc, ha, hb)])
+        np.testing.assert_array_equal(cnt[i], cp[0])
+        assert_wrench_close(W[i], Wp[0], rtol=1e-12)
+        ref = oracle_soft_rigid(sc)
+        np.testing.assert_array_equal(cnt[i], ref["counts"])
+        assert_wrench_close(W[i], ref["wrench"])
+    assert cnt[:, 0].sum() > 0
+    say("batched tune")
+    b.tune(reps=2)
+    b.step()
+    W2, cnt2 = b.get()
+    np.testing.assert_array_equal(cnt2, cnt)
+    for i in range(len(scs)):
+        assert_wrench_close(W2[i], W[i], rtol=1e-12)
+    # graph replays are bit-identical to each other
+    b.step()
+    W3, _ = b.get()
+    np.testing.assert_array_equal(W3, W2)
+    times = b.kernel_times(2)
+    assert times["query"] > 0
+    b.close()
+
+
+# ------------------------------------------------------------------ adjoint
+
+
+def _perturbed(X, d, h):
+    X = np.array(X, dtype=np.float64)
+    if d < 3:
+        from meshes import rot
+        R = X[:9].reshape(3, 3)
+        ax = np.zeros(3)
+        ax[d] = 1.0
+        X[:9] = (rot(ax, h) @ R).reshape(9)
+    else:
+        X[9 + d - 3] += h
+    return X
+
+
+def _fd_columns(engine, sc, ha, hb, h=1e-7, with_E=True):
+    base_W, base_c = engine.contact_step([pair_of(sc, ha, hb)])
+    cols = []
+    for d in range(6):
+        Wp, cp = engine.contact_step([pair_of(sc, ha, hb, _perturbed(sc["X_WB"], d, h))])
+        Wm, cm = engine.contact_step([pair_of(sc, ha, hb, _perturbed(sc["X_WB"], d, -h))])
+        if not (np.array_equal(cp, base_c) and np.array_equal(cm, base_c)):
+            cols.append(None)
+            continue
+        cols.append((Wp[0] - Wm[0]) / (2 * h))
+    return base_W[0], cols
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_contact_jacobian_matches_central_differences(engine, seed):
+    say("jacobian vs FD, differentiation env", seed)
+    sc = S.differentiation_env(0, seed)
+    ha, hb = add_scene(engine, sc)
+    J = engine.jacobian([pair_of(sc, ha, hb)])[0]
+    W, cols = _fd_columns(engine, sc, ha, hb)
+    checked = 0
+    for d, col in enumerate(cols):
+        if col is None:
+            continue
+        scale = np.abs(J[:, d]).max() + 1e-300
+        assert np.abs(J[:, d] - col).max() <= 1e-6 * scale + 1e-9 * np.abs(W).max(), (d, J[:, d], col)
+        checked += 1
+    assert checked >= 4
+    # pressures scale with E at fixed geometry: dW/dE = W / E
+    E = sc["soft"]["props"]["hydroelastic_modulus"]
+    np.testing.assert_allclose(J[:, 6], W / E, rtol=1e-9, atol=1e-12 * np.abs(W).max() / E)
+    # vector-Jacobian product
+    g = np.linspace(-1, 1, 12)
+    v = engine.backward([pair_of(sc, ha, hb)], g[None, :])[0]
+    np.testing.assert_allclose(v, J.T @ g, rtol=1e-12, atol=1e-12 * np.abs(J).max())
+
+
+def test_taxel_jacobian_matches_central_differences(engine):
+    say("taxel jacobian vs FD")
+    sc = S.differentiation_env(0, 3)
+    ha, hb = add_scene(engine, sc)
+    g = dict(sc["grid"])
+    grid = dict(geom=hb, X_W_sensor=sc["X_WB"], **g)
+    Jt = engine.taxel_jacobian(pair_of(sc, ha, hb), grid)
+    h = 1e-7
+
+    def taxels_at(XB):
+        engine.contact_step([pair_of(sc, ha, hb, XB)], keep_polygons=True)
+        gg = dict(grid)
+        gg["X_W_sensor"] = XB
+        return engine.taxels([gg])[0].reshape(-1)
+
+    base = taxels_at(sc["X_WB"])
+    assert np.abs(base).max() > 0
+    checked = 0
+    for d in (3, 4, 5, 2):
+        tp = taxels_at(_perturbed(sc["X_WB"], d, h))
+        tm = taxels_at(_perturbed(sc["X_WB"], d, -h))
+        fd = (tp - tm) / (2 * h)
+        err = np.abs(Jt[:, d] - fd)
+        scale = np.abs(Jt[:, d]).max() + np.abs(base).max() * 1e-3
+        ok = err <= 1e-6 * scale
+        # samples whose hit triangle changes under +-h are topology breaks
+        assert ok.mean() > 0.97, (d, err.max(), scale)
+        checked += 1
+    assert checked == 4
+    E = sc["soft"]["props"]["hydroelastic_modulus"]
+    np.testing.assert_allclose(Jt[:, 6], base / E, rtol=1e-9, atol=1e-12 * np.abs(base).max() / E)
+
+
+# ---------------------------------------------------------------- soft-soft
+
+
+def _soft_soft_small(seed):
+    return S.soft_soft_env(seed, 0, nt_range=(2e3, 6e3))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_soft_soft_parity(engine, seed):
+    say("soft-soft parity", seed)
+    sc = _soft_soft_small(seed)
+    ha, hb = add_scene(engine, sc)
+    a, b = sc["soft"], sc["soft_b"]
+    ref = O.soft_soft(a["V"], a["T"], a["p"], b["V"], b["T"], b["p"], sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"],
+                      a["props"]["hydroelastic_modulus"], a["props"]["dissipation"], a["props"]["dynamic_friction"],
+                      b["props"]["hydroelastic_modulus"], b["props"]["dissipation"], b["props"]["dynamic_friction"],
+                      a_is_M=a["geom_id"] < b["geom_id"])
+    W, cnt, (rec, key) = engine.contact_step([pair_of(sc, ha, hb)], face_capacity=1 << 18)
+    assert ref["counts"][1] > 0
+    np.testing.assert_array_equal(cnt[0], ref["counts"])
+    assert_wrench_close(W[0], ref["wrench"])
+    match_faces(rec, key, ref["faces"], ref["face_keys"])
+
+
+def test_soft_soft_jacobian(engine):
+    say("soft-soft jacobian vs FD")
+    sc = _soft_soft_small(3)
+    ha, hb = add_scene(engine, sc)
+    J = engine.jacobian([pair_of(sc, ha, hb)])[0]
+    W, cols = _fd_columns(engine, sc, ha, hb)
+    checked = 0
+    for d, col in enumerate(cols):
+        if col is None:
+            continue
+        scale = np.abs(J[:, d]).max() + 1e-300
+        assert np.abs(J[:, d] - col).max() <= 1e-6 * scale + 1e-9 * np.abs(W).max(), (d, J[:, d], col)
+        checked += 1
+    assert checked >= 3
