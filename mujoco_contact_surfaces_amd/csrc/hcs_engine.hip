@@ -816,7 +816,9 @@ int hcs_contact_step(hcs_ctx* h, int32_t npairs, const hcs_pair_in* in, const hc
 int hcs_candidates(hcs_ctx* h, const hcs_pair_in* pair, int64_t* keys, int64_t capacity, int64_t* count) {
   if (!h || !pair || !count || capacity < 0 || (capacity > 0 && !keys)) return HCS_ERR_ARG;
   int64_t dummy = 0;
-  return run_pairs(&h->c, 1, pair, nullptr, nullptr, nullptr, keys ? keys : &dummy, capacity, count);
+  /* keys == NULL: count-only query */
+  const int rc = run_pairs(&h->c, 1, pair, nullptr, nullptr, nullptr, keys ? keys : &dummy, keys ? capacity : 0, count);
+  return (!keys && rc == HCS_ERR_CAPACITY) ? HCS_OK : rc;
 }
 
 }  // extern "C"
