@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--plain-envs", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--config", default="", help="fixed launch shape: blocks_per_cu,queries_per_lane,tet_max")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the engine is single-device (multi-GPU sharding is design-only); "
@@ -141,7 +142,10 @@ def main():
     W0, cnt = batch.get()
     clips = int(cnt[:, 0].sum())
     log("step counts: clips %d polygons %d faces %d" % tuple(cnt.sum(axis=0)))
-    if not a.no_tune:
+    if a.config:
+        batch.set_config(*[int(v) for v in a.config.split(",")])
+        log("fixed config: %s" % batch.config())
+    elif not a.no_tune:
         ms = batch.tune(reps=2)
         log("autotuned: %s (%.3f ms)" % (batch.config(), ms))
 
@@ -176,7 +180,7 @@ def main():
     q_s = kt["query"] * 1e-3
     achieved = B / q_s
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, alg_bytes=B, kernel="k_query<double>", kernel_ms=kt["query"], **bparts)
+                traffic=None, alg_bytes=B, kernel="k_step_persistent", kernel_ms=kt["query"], **bparts)
     cpu = None if a.no_cpu else cpu_baseline(scs)
     value = clips / dt
     out = dict(metric=METRIC, value=value, unit="clips/s", n_gpus=1, steps=a.steps, warmup=a.warmup,

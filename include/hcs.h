@@ -171,8 +171,10 @@ int hcs_batch_step(hcs_batch* batch);
 int hcs_batch_get(hcs_batch* batch, hcs_pair_out* out);
 /* Autotuner: times the launch shapes on the current state, keeps the best. */
 int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
-/* Launch configuration: [block waves, persistent blocks per CU, items of tri-mode chunk] */
+/* Launch configuration: cfg = [persistent blocks per CU, queries per lane,
+ * largest rigid mesh handled in tet-query mode]. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
+int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
 /* Eager per-kernel timing with HIP events on hcs_stream: ms[0] prep,
  * ms[1] query/clip, ms[2] reduce; averages over reps. */
 int hcs_batch_kernel_times(hcs_batch* batch, int32_t reps, double* ms);

@@ -80,6 +80,7 @@ SIGNATURES = {
     "hcs_batch_get": (C.c_int, [_P, C.POINTER(PairOut)]),
     "hcs_batch_tune": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_batch_config": (C.c_int, [_P, _I32]),
+    "hcs_batch_set_config": (C.c_int, [_P, _I32]),
     "hcs_batch_kernel_times": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_batch_counts": (C.c_int, [_P, _I64]),
     "hcs_contact_jacobian": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), _D]),

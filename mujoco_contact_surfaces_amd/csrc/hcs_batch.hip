@@ -200,6 +200,14 @@ int hcs_batch_config(hcs_batch* b, int32_t* cfg) {
   return HCS_OK;
 }
 
+int hcs_batch_set_config(hcs_batch* b, const int32_t* cfg) {
+  if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0) return HCS_ERR_ARG;
+  b->blocks_per_cu = cfg[0], b->qpl = cfg[1], b->tet_max = cfg[2];
+  int rc = batch_plan(b);
+  if (!rc) rc = batch_capture(b);
+  return rc;
+}
+
 static int time_steps(hcs_batch* b, int reps, double* ms) {
   Ctx* c = b->c;
   hipEvent_t e0, e1;

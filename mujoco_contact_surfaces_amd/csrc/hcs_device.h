@@ -36,7 +36,7 @@ namespace hcs {
 
 struct Dual {
   double v, d;
-  __host__ __device__ Dual() : v(0), d(0) {}
+  Dual() = default;
   __host__ __device__ Dual(double a) : v(a), d(0) {}
   __host__ __device__ Dual(double a, double b) : v(a), d(b) {}
 };

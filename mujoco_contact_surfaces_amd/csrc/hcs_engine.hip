@@ -282,23 +282,27 @@ __global__ void k_prep(const EnvDesc* __restrict__ envs, const double* __restric
 template <class T>
 __global__ __launch_bounds__(64) void k_query(QueryArgs A, int dir_base, const double* dstar_dE, const double* invE,
                                               Partial* tpart) {
-  __shared__ WaveLds L;
+  __shared__ WaveLds<T> L;
   const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
                   A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan};
   const OutPtrs* op = (A.face_rec || A.poly_rec) ? &o : nullptr;
   const int dir = dir_base < 0 ? -1 : dir_base + (int)blockIdx.y;
   Partial* part = dir < 0 ? A.partials : tpart + (size_t)(dir - dir_base) * A.nitems;
-  if (A.counter == nullptr) {
-    const int it = blockIdx.x;
-    if (it < A.nitems) run_item<T>(L, A, it, dir, dstar_dE, invE, part + it, op);
-    return;
-  }
+  const int it = blockIdx.x;
+  if (it < A.nitems) run_item<T>(L, A, it, dir, dstar_dE, invE, part + it, op);
+}
+
+/* Persistent form of k_query for the batched step: every wave takes items
+ * from a global counter until none is left (the counter is reset by a fill
+ * kernel in the same graph). */
+__global__ __launch_bounds__(64) void k_step_persistent(QueryArgs A) {
+  __shared__ WaveLds<double> L;
   for (;;) {
     unsigned v = 0;
-    if (threadIdx.x == 0) v = atomicAdd(A.counter + (blockIdx.y), 1u);
+    if (threadIdx.x == 0) v = atomicAdd(A.counter, 1u);
     const int it = (int)__shfl(v, 0, 64);
     if (it >= A.nitems) break;
-    run_item<T>(L, A, it, dir, dstar_dE, invE, part + it, op);
+    run_item<double>(L, A, it, -1, nullptr, nullptr, A.partials + it, nullptr);
   }
 }
 
@@ -328,7 +332,10 @@ void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double
   if (n > 0) hipLaunchKernelGGL(k_prep, dim3((n + 127) / 128), dim3(128), 0, s, envs, state, xf, n);
 }
 void launch_query(hipStream_t s, const QueryArgs& a, int grid) {
-  if (grid > 0)
+  if (grid <= 0) return;
+  if (a.counter)
+    hipLaunchKernelGGL(k_step_persistent, dim3(grid), dim3(64), 0, s, a);
+  else
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_query<double>), dim3(grid), dim3(64), 0, s, a, -1, (const double*)nullptr,
                        (const double*)nullptr, (Partial*)nullptr);
 }

@@ -223,10 +223,7 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
 template <class T>
 __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic, int env, int a, int b, Acc<T>& acc,
                                          int64_t& np, int64_t& nf, const OutPtrs* o) {
-  const double* r = E.recA + (size_t)HCS_TREC * a;
-  double rr[HCS_TREC];
-#pragma unroll
-  for (int i = 0; i < 23; ++i) rr[i] = r[i];
+  const double* __restrict__ rr = E.recA + (size_t)HCS_TREC * a;
   T gA[3] = {ic.escale * rr[16], ic.escale * rr[17], ic.escale * rr[18]};
   T cA = ic.escale * rr[19];
   if (E.kind != KIND_SOFT) {
@@ -255,10 +252,7 @@ __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic,
     emit_polygon<T, 7>(ic, p, nS, gA, cA, zero3, acc, nf, o, env, a, b);
   } else {
     /* soft-soft: equal-pressure plane of tets a (A) and b (B) */
-    const double* rb = E.recB + (size_t)HCS_TREC * b;
-    double rbb[HCS_TREC];
-#pragma unroll
-    for (int i = 0; i < 23; ++i) rbb[i] = rb[i];
+    const double* __restrict__ rbb = E.recB + (size_t)HCS_TREC * b;
     T gbB[3] = {T(rbb[16]), T(rbb[17]), T(rbb[18])}, ghB[3] = {T(rbb[20]), T(rbb[21]), T(rbb[22])};
     T gb[3], gbh[3];
     rot(ic.X_AB, gbB, gb);
@@ -410,127 +404,206 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
   return v;
 }
 
-/* Shared-memory layout of one wave. */
-struct WaveLds {
-  int2 q[192];
-  int stk[HCS_STACK][64];
-  double tile[64 * 6];
-};
-
-/* Processes the top 64 (or all remaining) queued candidates. */
+/* Exact early rejection, evaluated on primal values before the full clip.
+ * Returns true when the candidate can produce a polygon.
+ * soft-rigid:
+ *  - the gradient filter of process_candidate (same arithmetic);
+ *  - all three triangle vertices farther than 1e-15 outside one tet face
+ *    plane: every vertex the clip creates is an affine combination of
+ *    them (rounding ~1e-17), so that stage leaves nothing;
+ *  - the tet's AABB farther than 1e-12 from the triangle's plane: the
+ *    clipped polygon stays within rounding of that plane and cannot meet
+ *    the tet.
+ * soft-soft: the equal-pressure plane, both gradient filters and the
+ * marching-tets sign test of process_candidate (same arithmetic). */
 template <class T>
-__device__ inline void drain(WaveLds& L, int& qn, bool all, const EnvDesc& E, const ItemCtx<T>& ic, int env,
-                             Acc<T>& acc, int64_t& nc, int64_t& np, int64_t& nf, const OutPtrs* o,
-                             const QueryArgs& A, int lane) {
-  while (qn >= 64 || (all && qn > 0)) {
-    __syncthreads();
-    const int take = qn >= 64 ? 64 : qn;
-    const int base = qn - take;
-    if (lane < take) {
-      int2 c = L.q[base + lane];
-      ++nc;
-      if (A.cand_key) {
-        unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
-        if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
-      } else {
-        process_candidate<T>(E, ic, env, c.x, c.y, acc, np, nf, o);
-      }
+__device__ inline bool survives(const EnvDesc& E, const ItemCtx<T>& ic, const double* X, int a, int b) {
+  const double* r = E.recA + (size_t)HCS_TREC * a;
+  if (E.kind != KIND_SOFT) {
+    const int32_t* f = E.FB + 3 * (size_t)b;
+    const double* nr = E.nrmB + 3 * (size_t)b;
+    double nS[3];
+    rot(X, nr, nS);
+    if (!(((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA)) return false;
+    double v[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xform(X, E.VB + 3 * (size_t)f[i], v[i]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double* n = r + 3 * k;
+      const double d = r[12 + k];
+      bool out = true;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) out = out && (((n[0] * v[i][0] + n[1] * v[i][1]) + n[2] * v[i][2]) + d > 1e-15);
+      if (out) return false;
     }
-    qn = base;
-    __syncthreads();
+    const double* bx = E.boxA + 6 * (size_t)a;
+    double dist = -((nS[0] * v[0][0] + nS[1] * v[0][1]) + nS[2] * v[0][2]);
+    double rad = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      dist += nS[c] * (0.5 * (bx[c] + bx[3 + c]));
+      rad += fabs(nS[c]) * (0.5 * (bx[3 + c] - bx[c]));
+    }
+    return !(fabs(dist) > rad + 1e-12);
   }
+  const double* rb = E.recB + (size_t)HCS_TREC * b;
+  const double sc = val(ic.escale);
+  double gA[3] = {sc * r[16], sc * r[17], sc * r[18]};
+  double cA = sc * r[19];
+  double gbB[3] = {rb[16], rb[17], rb[18]}, ghB[3] = {rb[20], rb[21], rb[22]};
+  double gb[3], gbh[3];
+  rot(X, gbB, gb);
+  rot(X, ghB, gbh);
+  double cb = rb[19] - dot3(gb, X + 9);
+  double nn[3] = {gA[0] - gb[0], gA[1] - gb[1], gA[2] - gb[2]};
+  double mag = sqrt(dot3(nn, nn));
+  if (!(mag > 0.0)) return false;
+  double nh[3] = {nn[0] / mag, nn[1] / mag, nn[2] / mag};
+  double off = -((cb - cA) / mag);
+  if (!(((nh[0] * r[20] + nh[1] * r[21]) + nh[2] * r[22]) > HCS_COS_ALPHA)) return false;
+  if (!(((-nh[0] * gbh[0] + -nh[1] * gbh[1]) + -nh[2] * gbh[2]) > HCS_COS_ALPHA)) return false;
+  const int32_t* ta = E.TA + 4 * (size_t)a;
+  int above = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double* v = E.VA + 3 * (size_t)ta[i];
+    above += (dot3(nh, v) + off) > 0;
+  }
+  return above != 0 && above != 4;
 }
 
-/* Body of the query kernel for one item. */
+/* Shared-memory layout of one wave: candidate queue (traversal output),
+ * survivor queue (input of the full clip), traversal stacks, a tile of
+ * rigid-triangle boxes for the tet-query mode, and the per-item context
+ * (poses, twists, material constants) read by every lane. */
 template <class T>
-__device__ inline void run_item(WaveLds& L, const QueryArgs& A, int it, int dir, const double* dstar_dE,
+struct WaveLds {
+  int2 q1[192];
+  int2 q2[128];
+  int stk[HCS_STACK][64];
+  double tile[64 * 6];
+  ItemCtx<T> ic;
+  double X[12];
+};
+
+/* Body of the query kernel for one item: one loop that alternates one
+ * production step (a traversal step, or one rigid triangle tested against
+ * every lane's tet) with the consumers (cheap tests on 64 queued candidates,
+ * full clip of 64 survivors), so each stage is instantiated once. */
+template <class T>
+__device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int dir, const double* dstar_dE,
                                 const double* invE, Partial* out, const OutPtrs* o) {
   const int lane = threadIdx.x & 63;
   const Item item = A.items[it];
   const int env = item.env;
   const EnvDesc E = A.envs[env];
   const double* st = A.state + (size_t)HCS_STATE * env;
-  ItemCtx<T> ic;
-  make_item_ctx<T>(E, st, dir, dstar_dE ? dstar_dE[env] : 0.0, invE ? invE[env] : 0.0, ic);
-  double X[12];
+  __syncthreads();
+  if (lane == 0) {
+    make_item_ctx<T>(E, st, dir, dstar_dE ? dstar_dE[env] : 0.0, invE ? invE[env] : 0.0, L.ic);
 #pragma unroll
-  for (int i = 0; i < 12; ++i) X[i] = val(ic.X_AB[i]);
+    for (int i = 0; i < 12; ++i) L.X[i] = val(L.ic.X_AB[i]);
+  }
+  __syncthreads();
+  const ItemCtx<T>& ic = L.ic;
+  const double* X = L.X;
   Acc<T> acc;
   acc.zero();
   int64_t nc = 0, np = 0, nf = 0;
-  int qn = 0;
-  if (E.kind == KIND_TET) {
-    /* queries are tets of A; the rigid triangles come through LDS tiles */
-    const int t = item.first + lane;
-    const bool valid = lane < item.count;
-    double bx[6];
+  int q1n = 0, q2n = 0;
+  const bool tet_mode = E.kind == KIND_TET;
+  /* producer state */
+  bool producing = true;
+  int sub = -1, sp = 0, visits = 0, b = 0;
+  bool valid = false;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  int t0 = 0, j = 0, ntile = 0;
+  double bx[6] = {0, 0, 0, 0, 0, 0};
+  if (tet_mode) {
+    b = item.first + lane; /* the lane's tet */
+    valid = lane < item.count;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) bx[c] = valid ? E.boxA[6 * (size_t)t + c] : 0.0;
-    for (int t0 = 0; t0 < E.nB; t0 += 64) {
-      const int nt = (E.nB - t0) < 64 ? (E.nB - t0) : 64;
-      __syncthreads();
-      if (lane < nt) {
-        const int32_t* f = E.FB + 3 * (size_t)(t0 + lane);
-        double lo[3], hi[3];
+    for (int c = 0; c < 6; ++c) bx[c] = valid ? E.boxA[6 * (size_t)b + c] : 0.0;
+  }
+  for (;;) {
+    /* ---- produce */
+    if (producing) {
+      if (tet_mode) {
+        if (j == ntile) {
+          if (sub >= 0) t0 += 64;
+          sub = 0;
+          if (t0 >= E.nB) {
+            producing = false;
+          } else {
+            ntile = (E.nB - t0) < 64 ? (E.nB - t0) : 64;
+            j = 0;
+            __syncthreads();
+            if (lane < ntile) {
+              const int32_t* f = E.FB + 3 * (size_t)(t0 + lane);
+              double l3[3], h3[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          double y[3];
-          const double* v = E.VB + 3 * (size_t)f[i];
-          xform(X, v, y);
+              for (int i = 0; i < 3; ++i) {
+                double y[3];
+                xform(X, E.VB + 3 * (size_t)f[i], y);
 #pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
-            hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+                for (int c = 0; c < 3; ++c) {
+                  l3[c] = i == 0 ? y[c] : fmin(l3[c], y[c]);
+                  h3[c] = i == 0 ? y[c] : fmax(h3[c], y[c]);
+                }
+              }
+#pragma unroll
+              for (int c = 0; c < 3; ++c) L.tile[6 * lane + c] = l3[c], L.tile[6 * lane + 3 + c] = h3[c];
+            }
+            __syncthreads();
           }
         }
+        if (producing) {
+          const double* tb = &L.tile[6 * j];
+          const bool hit = valid && box_overlap(bx, tb, tb + 3);
+          const unsigned long long m = __ballot(hit);
+          if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, t0 + j);
+          q1n += __popcll(m);
+          ++j;
+        }
+      } else if (!__any(sp > 0)) {
+        /* next query of this lane, or done */
+        ++sub;
+        if (sub >= A.queries_per_lane) {
+          producing = false;
+        } else {
+          const int qi = sub * 64 + lane;
+          valid = qi < item.count;
+          b = item.first + qi;
+          if (valid) {
+            const int nvq = E.kind == KIND_SOFT ? 4 : 3;
+            const int32_t* f = E.FB + (size_t)nvq * b;
+            for (int i = 0; i < nvq; ++i) {
+              double y[3];
+              xform(X, E.VB + 3 * (size_t)f[i], y);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) L.tile[6 * lane + c] = lo[c], L.tile[6 * lane + 3 + c] = hi[c];
-      }
-      __syncthreads();
-      for (int j = 0; j < nt; ++j) {
-        const double* tb = &L.tile[6 * j];
-        const bool hit = valid && box_overlap(bx, tb, tb + 3);
-        const unsigned long long m = __ballot(hit);
-        if (hit) L.q[qn + lane_prefix(m)] = make_int2(t, t0 + j);
-        qn += __popcll(m);
-        drain<T>(L, qn, false, E, ic, env, acc, nc, np, nf, o, A, lane);
-      }
-      drain<T>(L, qn, true, E, ic, env, acc, nc, np, nf, o, A, lane);
-    }
-  } else {
-    for (int sub = 0; sub < A.queries_per_lane; ++sub) {
-      const int qi = sub * 64 + lane;
-      const bool valid = qi < item.count;
-      const int b = item.first + qi;
-      double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-      if (valid) {
-        const int nvq = E.kind == KIND_SOFT ? 4 : 3;
-        const int32_t* f = E.FB + (size_t)nvq * b;
-        for (int i = 0; i < nvq; ++i) {
-          double y[3];
-          xform(X, E.VB + 3 * (size_t)f[i], y);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
-            hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+              for (int c = 0; c < 3; ++c) {
+                lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
+                hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+              }
+            }
+          }
+          visits = 0;
+          sp = 0;
+          if (E.ntA == 1) {
+            const bool hit = valid && box_overlap(E.boxA, lo, hi);
+            const unsigned long long m = __ballot(hit);
+            if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(0, b);
+            q1n += __popcll(m);
+          } else if (valid) {
+            L.stk[0][lane] = 0;
+            sp = 1;
           }
         }
-      }
-      int sp = 0;
-      if (E.ntA == 1) {
-        const bool hit = valid && box_overlap(E.boxA, lo, hi);
-        const unsigned long long m = __ballot(hit);
-        if (hit) L.q[qn + lane_prefix(m)] = make_int2(0, b);
-        qn += __popcll(m);
-        drain<T>(L, qn, false, E, ic, env, acc, nc, np, nf, o, A, lane);
-      } else if (valid) {
-        L.stk[0][lane] = 0;
-        sp = 1;
-      }
-      /* A tree of ntA leaves has ntA-1 internal nodes, each popped at most
-       * once per query; the cap only guards against a malformed tree. */
-      int visits = 0;
-      while (__any(sp > 0)) {
+      } else {
+        /* one traversal step; a tree of ntA leaves has ntA-1 internal
+         * nodes, each popped at most once per query (the cap only guards
+         * against a malformed tree) */
         int c0 = -1, c1 = -1;
         if (sp > 0 && ++visits > E.ntA) {
           atomicOr(A.error, 2);
@@ -556,13 +629,50 @@ __device__ inline void run_item(WaveLds& L, const QueryArgs& A, int it, int dir,
           }
         }
         const unsigned long long m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
-        if (c0 >= 0) L.q[qn + lane_prefix(m0)] = make_int2(c0, b);
-        if (c1 >= 0) L.q[qn + __popcll(m0) + lane_prefix(m1)] = make_int2(c1, b);
-        qn += __popcll(m0) + __popcll(m1);
-        drain<T>(L, qn, false, E, ic, env, acc, nc, np, nf, o, A, lane);
+        if (c0 >= 0) L.q1[q1n + lane_prefix(m0)] = make_int2(c0, b);
+        if (c1 >= 0) L.q1[q1n + __popcll(m0) + lane_prefix(m1)] = make_int2(c1, b);
+        q1n += __popcll(m0) + __popcll(m1);
       }
     }
-    drain<T>(L, qn, true, E, ic, env, acc, nc, np, nf, o, A, lane);
+    /* ---- consume until both queues hold fewer than 64 (or are empty at
+     * the end); q1 < 192 and q2 < 128 hold throughout */
+    for (;;) {
+      const bool do2 = q2n >= 64 || (!producing && q1n == 0 && q2n > 0);
+      const bool do1 = q1n >= 64 || (!producing && q1n > 0);
+      if (!do1 && !do2) break;
+      __syncthreads();
+      if (do2) {
+        const int take = q2n >= 64 ? 64 : q2n;
+        const int base = q2n - take;
+        if (lane < take) {
+          const int2 c = L.q2[base + lane];
+          process_candidate<T>(E, ic, env, c.x, c.y, acc, np, nf, o);
+        }
+        q2n = base;
+      } else {
+        const int take = q1n >= 64 ? 64 : q1n;
+        const int base = q1n - take;
+        bool keep = false;
+        int2 c = make_int2(0, 0);
+        if (lane < take) {
+          c = L.q1[base + lane];
+          ++nc;
+          if (A.cand_key) {
+            unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
+            if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
+          } else {
+            keep = survives<T>(E, ic, X, c.x, c.y);
+          }
+        }
+        q1n = base;
+        const unsigned long long m = __ballot(keep);
+        __syncthreads();
+        if (keep) L.q2[q2n + lane_prefix(m)] = c;
+        q2n += __popcll(m);
+      }
+      __syncthreads();
+    }
+    if (!producing) break;
   }
   /* fixed-order wave reduction */
   Partial r;

@@ -264,6 +264,10 @@ class Batch:
         self.e._chk(self.e.lib.hcs_batch_config(self.b, A.i32ptr(cfg)))
         return dict(blocks_per_cu=int(cfg[0]), queries_per_lane=int(cfg[1]), tet_query_max_tris=int(cfg[2]))
 
+    def set_config(self, blocks_per_cu, queries_per_lane, tet_query_max_tris):
+        cfg = np.array([blocks_per_cu, queries_per_lane, tet_query_max_tris], dtype=np.int32)
+        self.e._chk(self.e.lib.hcs_batch_set_config(self.b, A.i32ptr(cfg)))
+
     def kernel_times(self, reps=5):
         ms = np.zeros(3)
         self.e._chk(self.e.lib.hcs_batch_kernel_times(self.b, int(reps), A.dptr(ms)))
