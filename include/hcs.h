@@ -178,6 +178,11 @@ int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
 /* Eager per-kernel timing with HIP events on hcs_stream: ms[0] prep,
  * ms[1] query/clip, ms[2] reduce; averages over reps. */
 int hcs_batch_kernel_times(hcs_batch* batch, int32_t reps, double* ms);
+/* Per-item profile of one eager step (shader clock): out (capacity x 8) =
+ * [pair, kind (0 triangle queries, 1 tet queries, 2 soft-soft), queries,
+ * cycles, production steps, candidates, cheap-test survivors, n elements of
+ * B]; *nitems = number of items (capacity 0 queries it). */
+int hcs_batch_item_stats(hcs_batch* batch, int64_t* out, int64_t capacity, int64_t* nitems);
 /* Work-unit counts of the last step: [candidates, polygons, faces]. */
 int hcs_batch_counts(hcs_batch* batch, int64_t* counts);
 

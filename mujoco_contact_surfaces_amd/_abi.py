@@ -83,6 +83,7 @@ SIGNATURES = {
     "hcs_batch_set_config": (C.c_int, [_P, _I32]),
     "hcs_batch_kernel_times": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_batch_counts": (C.c_int, [_P, _I64]),
+    "hcs_batch_item_stats": (C.c_int, [_P, _I64, C.c_int64, _I64]),
     "hcs_contact_jacobian": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), _D]),
     "hcs_contact_backward": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), _D, _D]),
     "hcs_taxel_jacobian": (C.c_int, [_P, C.POINTER(PairIn), C.POINTER(StepOpts), C.POINTER(TaxelGrid), _D]),

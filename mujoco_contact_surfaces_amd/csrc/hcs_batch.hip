@@ -255,6 +255,36 @@ int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
   return rc;
 }
 
+int hcs_batch_item_stats(hcs_batch* b, int64_t* out, int64_t capacity, int64_t* nitems) {
+  if (!b || !nitems || capacity < 0 || (capacity > 0 && !out)) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  const int n = (int)b->plan.items.size();
+  *nitems = n;
+  if (capacity < n) return capacity == 0 ? HCS_OK : set_err(c, HCS_ERR_CAPACITY, "item stats buffer too small");
+  DBuf d;
+  int rc = ensure(c, d, sizeof(unsigned long long) * 4 * std::max(1, n));
+  if (rc) return rc;
+  QueryArgs a = batch_args(b);
+  a.item_stats = (unsigned long long*)d.ptr;
+  launch_fill_u32(c->stream, a.counter, 0u, 1);
+  launch_prep(c->stream, a.envs, a.state, (double*)b->xf.ptr, b->npairs);
+  launch_query(c->stream, a, grid_of(b));
+  std::vector<unsigned long long> h(4 * (size_t)n);
+  rc = hip_check(c, hipGetLastError(), "stats step");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(h.data(), d.ptr, h.size() * 8, hipMemcpyDeviceToHost, c->stream), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  (void)hipFree(d.ptr);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) {
+    const Item& it = b->plan.items[i];
+    int64_t* o = out + 8 * (size_t)i;
+    o[0] = it.env, o[1] = b->plan.envs[it.env].kind, o[2] = it.count;
+    o[3] = (int64_t)h[4 * i], o[4] = (int64_t)h[4 * i + 1], o[5] = (int64_t)h[4 * i + 2], o[6] = (int64_t)h[4 * i + 3];
+    o[7] = b->plan.envs[it.env].nB;
+  }
+  return HCS_OK;
+}
+
 int hcs_batch_kernel_times(hcs_batch* b, int32_t reps, double* ms) {
   if (!b || !ms || reps <= 0) return HCS_ERR_ARG;
   Ctx* c = b->c;

@@ -377,7 +377,9 @@ static int build_bvh(Ctx* c, Mesh& m) {
   m.height = 0;
   if (n == 1) {
     HCHK(c, hipMalloc((void**)&m.nodes, sizeof(Node)));
-    return HCS_OK;
+    const int32_t zero = 0;
+    int rc = upload(c, (void**)&m.leaf, &zero, sizeof(int32_t));
+    return rc ? rc : hip_check(c, hipStreamSynchronize(c->stream), "sync");
   }
   /* Scene bounds from the vertices (they bound every tet AABB). */
   std::vector<double> Vh(3 * (size_t)m.nv);
@@ -428,6 +430,10 @@ static int build_bvh(Ctx* c, Mesh& m) {
     fail(hipStreamSynchronize(c->stream), "sync");
     m.height = h;
   }
+  if (rc == HCS_OK) {
+    m.leaf = idx2; /* kept: Morton order of the tets */
+    idx2 = nullptr;
+  }
   for (void* ptr : {(void*)codes, (void*)codes2, (void*)idx, (void*)idx2, (void*)np, (void*)lp, (void*)flags,
                     (void*)rh, tmp})
     if (ptr) (void)hipFree(ptr);
@@ -437,7 +443,8 @@ static int build_bvh(Ctx* c, Mesh& m) {
 }
 
 static void free_mesh(Mesh& m) {
-  for (void* p : {(void*)m.V, (void*)m.E, (void*)m.p, (void*)m.rec, (void*)m.box, (void*)m.nodes, (void*)m.nrm})
+  for (void* p : {(void*)m.V, (void*)m.E, (void*)m.p, (void*)m.rec, (void*)m.box, (void*)m.nodes, (void*)m.nrm,
+                  (void*)m.leaf})
     if (p) (void)hipFree(p);
   m = Mesh();
 }
@@ -458,7 +465,7 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
       return set_err(c, HCS_ERR_ARG,
                      "pair " + std::to_string(e) + ": geom_a must be soft (pass soft-rigid pairs soft first)");
     EnvDesc& E = plan.envs[e];
-    E.recA = A.rec, E.boxA = A.box, E.nodesA = A.nodes, E.TA = A.E, E.VA = A.V, E.ntA = A.nelem;
+    E.recA = A.rec, E.boxA = A.box, E.nodesA = A.nodes, E.leafA = A.leaf, E.TA = A.E, E.VA = A.V, E.ntA = A.nelem;
     E.VB = B.V, E.FB = B.E, E.nrmB = B.nrm, E.recB = B.rec, E.nB = B.nelem, E.nvB = B.nv;
     E.b_soft = B.kind == 1;
     E.a_is_M = A.geom_id < B.geom_id;

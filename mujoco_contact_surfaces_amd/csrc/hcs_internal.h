@@ -23,6 +23,7 @@ struct Mesh {
   double* rec = nullptr;
   double* box = nullptr;
   Node* nodes = nullptr;
+  int32_t* leaf = nullptr; /* tets in Morton (BVH leaf) order */
   double* nrm = nullptr;
 };
 

@@ -9,6 +9,7 @@
  *               box   nt x 6   AABB lo, hi (soft frame)
  *               nodes (nt-1) x Node   binary LBVH, child boxes stored in the
  *                               parent so one 112-byte fetch tests both
+ *               leaf  nt int32 tets in Morton order
  *               T     nt x 4 int32, V nv x 3, p nv
  *   rigid mesh: V nv x 3, F ntri x 3 int32, nrm ntri x 3 (unit normals)
  * Per step:    state  npairs x 36 (X_WA, X_WB, V_WA, V_WB), xf npairs x 12
@@ -39,6 +40,7 @@ struct EnvDesc {
   const double* recA;
   const double* boxA;
   const Node* nodesA;
+  const int32_t* leafA; /* A's tets in Morton order (tet-query items) */
   const int32_t* TA;
   const double* VA;
   const double* VB;
@@ -88,6 +90,9 @@ struct QueryArgs {
   unsigned long long* cand_cnt;
   int64_t cand_cap;
   int* error;
+  /* optional per-item profile: cycles, production steps, candidates,
+   * survivors of the cheap tests (4 x uint64 per item) */
+  unsigned long long* item_stats;
 };
 
 }  // namespace hcs

@@ -398,6 +398,16 @@ __device__ inline Dual wave_sum<Dual>(Dual v) {
   }
   return v;
 }
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
 __device__ inline int64_t wave_sum_i(int64_t v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -495,6 +505,8 @@ template <class T>
 __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int dir, const double* dstar_dE,
                                 const double* invE, Partial* out, const OutPtrs* o) {
   const int lane = threadIdx.x & 63;
+  const unsigned long long t_begin = A.item_stats ? clock64() : 0ull;
+  unsigned long long steps = 0, survivors = 0;
   const Item item = A.items[it];
   const int env = item.env;
   const EnvDesc E = A.envs[env];
@@ -518,26 +530,38 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
   int sub = -1, sp = 0, visits = 0, b = 0;
   bool valid = false;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  int t0 = 0, j = 0, ntile = 0;
-  double bx[6] = {0, 0, 0, 0, 0, 0};
+  int t0 = 0, ntile = 0;
+  unsigned long long tmask = 0;
+  bool tile_loaded = false;
+  double bx[6] = {0, 0, 0, 0, 0, 0}, ulo[3] = {0, 0, 0}, uhi[3] = {0, 0, 0};
   if (tet_mode) {
-    b = item.first + lane; /* the lane's tet */
+    /* the lane's tet: 64 consecutive leaves in Morton order, so the item's
+     * union box is compact and culls most rigid triangles */
     valid = lane < item.count;
+    b = valid ? E.leafA[item.first + lane] : 0;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) bx[c] = valid ? E.boxA[6 * (size_t)b + c] : 0.0;
+    for (int c = 0; c < 3; ++c) {
+      bx[c] = valid ? E.boxA[6 * (size_t)b + c] : __builtin_inf();
+      bx[3 + c] = valid ? E.boxA[6 * (size_t)b + 3 + c] : -__builtin_inf();
+      ulo[c] = wave_min(bx[c]);
+      uhi[c] = wave_max(bx[3 + c]);
+    }
   }
   for (;;) {
     /* ---- produce */
+    ++steps;
     if (producing) {
       if (tet_mode) {
-        if (j == ntile) {
-          if (sub >= 0) t0 += 64;
-          sub = 0;
+        if (tmask == 0) {
+          /* next tile of 64 rigid triangles: boxes into LDS, keep those that
+           * touch the item's union box */
+          if (tile_loaded) t0 += 64;
+          tile_loaded = true;
           if (t0 >= E.nB) {
             producing = false;
           } else {
             ntile = (E.nB - t0) < 64 ? (E.nB - t0) : 64;
-            j = 0;
+            bool touch = false;
             __syncthreads();
             if (lane < ntile) {
               const int32_t* f = E.FB + 3 * (size_t)(t0 + lane);
@@ -554,17 +578,20 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
               }
 #pragma unroll
               for (int c = 0; c < 3; ++c) L.tile[6 * lane + c] = l3[c], L.tile[6 * lane + 3 + c] = h3[c];
+              const double ub[6] = {ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2]};
+              touch = box_overlap(ub, l3, h3);
             }
+            tmask = __ballot(touch);
             __syncthreads();
           }
-        }
-        if (producing) {
-          const double* tb = &L.tile[6 * j];
+        } else {
+          const int jj = __builtin_ctzll(tmask);
+          tmask &= tmask - 1;
+          const double* tb = &L.tile[6 * jj];
           const bool hit = valid && box_overlap(bx, tb, tb + 3);
           const unsigned long long m = __ballot(hit);
-          if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, t0 + j);
+          if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, t0 + jj);
           q1n += __popcll(m);
-          ++j;
         }
       } else if (!__any(sp > 0)) {
         /* next query of this lane, or done */
@@ -669,6 +696,7 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
         __syncthreads();
         if (keep) L.q2[q2n + lane_prefix(m)] = c;
         q2n += __popcll(m);
+        survivors += __popcll(m);
       }
       __syncthreads();
     }
@@ -693,6 +721,10 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
     }
     r.nc = snc, r.np = snp, r.nf = snf;
     *out = r;
+    if (A.item_stats) {
+      unsigned long long* q = A.item_stats + 4 * (size_t)it;
+      q[0] = clock64() - t_begin, q[1] = steps, q[2] = (unsigned long long)snc, q[3] = survivors;
+    }
   }
 }
 

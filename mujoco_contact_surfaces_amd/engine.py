@@ -273,6 +273,15 @@ class Batch:
         self.e._chk(self.e.lib.hcs_batch_kernel_times(self.b, int(reps), A.dptr(ms)))
         return dict(prep=ms[0], query=ms[1], reduce=ms[2])
 
+    def item_stats(self):
+        """Per-item profile of one step: columns pair, kind, queries, cycles,
+        production steps, candidates, survivors, elements of B."""
+        n = C.c_int64()
+        self.e._chk(self.e.lib.hcs_batch_item_stats(self.b, None, 0, C.byref(n)))
+        out = np.zeros((max(1, n.value), 8), dtype=np.int64)
+        self.e._chk(self.e.lib.hcs_batch_item_stats(self.b, A.i64ptr(out), out.shape[0], C.byref(n)))
+        return out[:n.value]
+
     def counts(self):
         c = np.zeros(3, dtype=np.int64)
         self.e._chk(self.e.lib.hcs_batch_counts(self.b, A.i64ptr(c)))
