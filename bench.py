@@ -8,7 +8,9 @@ against a rigid mesh (triangle count log-uniform over 12-20k; box, plane grid
 or icosphere) at 0.5-8 mm penetration.  One step = poses and twists in (already
 resident in HBM), per-environment net wrenches out: LBVH traversal, candidate
 compaction, clipping, face evaluation, Hunt-Crossley/friction force law and
-deterministic reductions, replayed as one hipGraph.
+deterministic reductions, replayed as one hipGraph: k_cand (LBVH traversal or
+tiled tet queries, exact cheap rejections, survivor compaction), a scan,
+k_clip (clip, faces, force law per survivor) and a fixed-order reduction.
 
 Metric: tet-tri clips per second, a clip being one element of the canonical
 candidate set (tet and triangle AABBs overlap in the soft frame).  Printed on
@@ -51,21 +53,28 @@ def build_workload(engine, n_env, seed):
     return scs, np.array(handles, dtype=np.int32), state
 
 
-def algorithmic_bytes(engine, scs, handles, counts):
-    """B_alg per step: rigid meshes read once, distinct candidate tets and
-    triangles, non-empty polygons, pair state in and wrenches out.  The fused
-    kernel never materialises the candidate list, so no per-candidate term."""
+def algorithmic_bytes(engine, scs, handles, counts, survivors, nitems):
+    """Algorithmic HBM bytes per step of the two pipeline kernels.
+
+    k_cand (candidate generation + exact cheap rejections): every rigid mesh
+    once (24 B per vertex, 12 B per triangle index triple, 24 B per triangle
+    normal), each distinct candidate tet once (192 B record + 48 B box), the
+    pair state (288 B) and work item (16 B) in, 16 B per survivor out.
+    k_clip: 16 B per survivor in, one 80 B contribution out; element reads are
+    left out, so this is a lower bound."""
     T_C = R_C = 0
     for sc, (ha, hb) in zip(scs, handles):
         keys = engine.candidates((ha, hb, sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]))
         ntri = sc["rigid"]["F"].shape[0]
         T_C += np.unique(keys // ntri).size
         R_C += np.unique(keys % ntri).size
-    rigid = sum(24 * sc["rigid"]["V"].shape[0] + 12 * sc["rigid"]["F"].shape[0] for sc in scs)
-    P = int(counts[:, 1].sum())
+    rigid = sum(24 * sc["rigid"]["V"].shape[0] + 36 * sc["rigid"]["F"].shape[0] for sc in scs)
     E = len(scs)
-    B = rigid + 144 * T_C + 84 * R_C + 48 * P + 384 * E
-    return B, dict(rigid_bytes=rigid, distinct_tets=T_C, distinct_tris=R_C, polygons=P)
+    b_cand = rigid + 240 * T_C + 288 * E + 16 * nitems + 16 * survivors
+    b_clip = 96 * survivors
+    return dict(k_cand=b_cand, k_clip=b_clip), dict(rigid_bytes=rigid, distinct_tets=T_C, distinct_tris=R_C,
+                                                    survivors=survivors, items=nitems,
+                                                    polygons=int(counts[:, 1].sum()))
 
 
 def cpu_baseline(scs, budget_s=12.0):
@@ -176,18 +185,22 @@ def main():
         pclips += int(c[0, 0])
     plain = pclips / (time.perf_counter() - t1)
 
-    B, bparts = algorithmic_bytes(engine, scs, handles, cnt)
-    q_s = kt["query"] * 1e-3
-    achieved = B / q_s
+    st = batch.item_stats()
+    survivors = int(st[:, 6].sum())
+    B, bparts = algorithmic_bytes(engine, scs, handles, cnt, survivors, st.shape[0])
+    dom = "k_cand" if kt["cand"] >= kt["clip"] else "k_clip"
+    k_ms = kt["cand"] if dom == "k_cand" else kt["clip"]
+    achieved = B[dom] / (k_ms * 1e-3)
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, alg_bytes=B, kernel="k_step_persistent", kernel_ms=kt["query"], **bparts)
+                traffic=None, kernel=dom, kernel_ms=k_ms, alg_bytes=B[dom], kernel_ms_all=kt,
+                alg_bytes_all=B, **bparts)
     cpu = None if a.no_cpu else cpu_baseline(scs)
     value = clips / dt
     out = dict(metric=METRIC, value=value, unit="clips/s", n_gpus=1, steps=a.steps, warmup=a.warmup,
                ms_per_step=dt * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
                data="synthetic",
                config=dict(workload="batched: %d envs, soft tets log-uniform 2k-100k, rigid tris log-uniform 12-20k, "
-                                    "0.5-8 mm penetration, one persistent hipGraph step" % a.envs,
+                                    "0.5-8 mm penetration, one hipGraph step" % a.envs,
                            envs=a.envs, total_tets=total_tets, total_tris=total_tris, clips_per_step=clips,
                            polygons_per_step=int(cnt[:, 1].sum()), faces_per_step=int(cnt[:, 2].sum()),
                            autotune=batch.config(), parallelism="single GPU"),

@@ -175,8 +175,9 @@ int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
  * largest rigid mesh handled in tet-query mode]. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
 int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
-/* Eager per-kernel timing with HIP events on hcs_stream: ms[0] prep,
- * ms[1] query/clip, ms[2] reduce; averages over reps. */
+/* Eager per-kernel timing with HIP events on hcs_stream, averaged over
+ * reps: ms[0] candidate kernel (traversal, cheap tests, compaction),
+ * ms[1] clip/force kernel, ms[2] scan + per-pair reduction. */
 int hcs_batch_kernel_times(hcs_batch* batch, int32_t reps, double* ms);
 /* Per-item profile of one eager step (shader clock): out (capacity x 8) =
  * [pair, kind (0 triangle queries, 1 tet queries, 2 soft-soft), queries,

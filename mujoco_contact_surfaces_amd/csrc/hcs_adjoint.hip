@@ -2,15 +2,15 @@
  * hcs_adjoint.hip -- differentiation of the contact step at fixed clipping
  * topology (no reference counterpart).
  *
- * Tangent-linear recomputation: the query kernel is re-run with the Dual
- * scalar type, one tangent direction per grid.y slice (rotation and
- * translation of geometry B's pose, modulus of geometry A), recomputing the
- * polygons instead of storing them.  Candidate pairs, inside/outside tests,
- * filters and branches follow the primal values, so the result is the exact
- * derivative of the piecewise-smooth map on the current piece.  The
- * vector-Jacobian product (the adjoint) is J^T applied to the output
- * cotangent; with 7 inputs and 12 (+ taxel) outputs the tangent form is the
- * cheaper way to get it.
+ * Tangent-linear recomputation with no stored polygons: the candidate kernel
+ * runs once on primal values (same survivors as the forward step), then the
+ * clip kernel is instantiated with the Dual scalar type and run once per
+ * tangent direction (grid.y): rotation and translation of geometry B's pose,
+ * and the modulus of geometry A.  Inside/outside tests, filters and branches
+ * follow the primal values, so the result is the exact derivative of the
+ * piecewise-smooth map on the current piece.  The vector-Jacobian product
+ * (the adjoint) is J^T applied to the output cotangent; with 7 inputs and 12
+ * (+ taxel) outputs the tangent form is the cheaper way to get it.
  */
 #include <hip/hip_runtime.h>
 
@@ -31,77 +31,74 @@ using namespace hcs;
 namespace hcs {
 
 static const int kDirs = 7;
-/* The fused tangent pass (k_query<Dual>) ran for more than 180 s on its
- * first gfx950 run without finishing; a CPU emulation of the same source
- * finishes and matches central differences.  Until the cause is known the
- * path is off and the entry points return HCS_ERR_STATE. */
-static const bool kFusedTangentEnabled = false;
 
-/* Uploads plan and state of npairs pairs into the context scratch. */
-int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan) {
-  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
-  if (rc) return rc;
-  std::vector<int32_t> geoms(2 * (size_t)npairs);
-  std::vector<double> state((size_t)HCS_STATE * npairs);
-  for (int e = 0; e < npairs; ++e) {
-    geoms[2 * e] = in[e].geom_a, geoms[2 * e + 1] = in[e].geom_b;
-    double* s = &state[(size_t)HCS_STATE * e];
-    std::memcpy(s, in[e].X_WA, 12 * sizeof(double));
-    std::memcpy(s + 12, in[e].X_WB, 12 * sizeof(double));
-    std::memcpy(s + 24, in[e].V_WA, 6 * sizeof(double));
-    std::memcpy(s + 30, in[e].V_WB, 6 * sizeof(double));
-  }
-  rc = build_plan(c, npairs, geoms.data(), opts, 1, c->tet_mode_max_tris, plan);
-  if (rc) return rc;
-  const size_t nitems = std::max<size_t>(1, plan.items.size());
-  if ((rc = ensure(c, c->d_envs, sizeof(EnvDesc) * npairs)) || (rc = ensure(c, c->d_state, state.size() * 8)) ||
-      (rc = ensure(c, c->d_xf, 12 * sizeof(double) * npairs)) || (rc = ensure(c, c->d_items, sizeof(Item) * nitems)) ||
-      (rc = ensure(c, c->d_partials, sizeof(Partial) * nitems * kDirs)) || (rc = ensure(c, c->d_err, 64)) ||
-      (rc = ensure(c, c->d_poly_cnt, 64)))
-    return rc;
-  rc = hip_check(c, hipMemcpyAsync(c->d_envs.ptr, plan.envs.data(), sizeof(EnvDesc) * npairs, hipMemcpyHostToDevice,
-                                   c->stream),
-                 "upload");
-  if (!rc)
-    rc = hip_check(c, hipMemcpyAsync(c->d_state.ptr, state.data(), state.size() * 8, hipMemcpyHostToDevice, c->stream),
-                   "upload");
-  if (!rc && !plan.items.empty())
-    rc = hip_check(c, hipMemcpyAsync(c->d_items.ptr, plan.items.data(), sizeof(Item) * plan.items.size(),
-                                     hipMemcpyHostToDevice, c->stream),
-                   "upload");
-  if (!rc) rc = hip_check(c, hipMemsetAsync(c->d_err.ptr, 0, 64, c->stream), "memset");
-  return rc;
-}
+/* The tangent pass stays switched off: an earlier fused tangent kernel went
+ * 180 s without finishing on gfx950 and its cause has not been shown, so
+ * every differentiation entry point returns HCS_ERR_STATE until it is.  The
+ * code below (forward candidate kernel + Dual clip kernel) is exercised on
+ * the CPU only. */
+static const bool kTangentEnabled = false;
 
-/* Per-pair tangent seeds: 1/E_A and d(d*)/dE_A. */
-static void seeds(Ctx* c, const hcs_pair_in* in, int npairs, std::vector<double>& invE, std::vector<double>& ddE) {
-  invE.assign(npairs, 0.0);
-  ddE.assign(npairs, 0.0);
+/* Per-pair tangent seeds: d(d*)/dE_A then 1/E_A. */
+static void seeds(Ctx* c, const hcs_pair_in* in, int npairs, std::vector<double>& sd) {
+  sd.assign(2 * (size_t)npairs, 0.0);
   for (int e = 0; e < npairs; ++e) {
     const Mesh& A = c->meshes[in[e].geom_a];
     const Mesh& B = c->meshes[in[e].geom_b];
     const double EA = A.props.hydroelastic_modulus;
-    invE[e] = 1.0 / EA;
+    sd[npairs + e] = 1.0 / EA;
     if (B.kind == 1) {
       /* d* = E_B/(E_A+E_B) d_A + E_A/(E_A+E_B) d_B (plugin.cpp:138-159) */
       const double EB = B.props.hydroelastic_modulus, s = EA + EB;
-      ddE[e] = EB * (B.props.dissipation - A.props.dissipation) / (s * s);
+      sd[e] = EB * (B.props.dissipation - A.props.dissipation) / (s * s);
     }
   }
 }
 
-static QueryArgs tangent_args(Ctx* c, const Plan& plan) {
-  QueryArgs a{};
-  a.envs = (const EnvDesc*)c->d_envs.ptr;
-  a.state = (const double*)c->d_state.ptr;
-  a.xf = (const double*)c->d_xf.ptr;
-  a.items = (const Item*)c->d_items.ptr;
-  a.nitems = (int)plan.items.size();
-  a.queries_per_lane = 1;
-  a.counter = nullptr;
-  a.partials = (Partial*)c->d_partials.ptr;
-  a.error = (int*)c->d_err.ptr;
-  return a;
+/* One tangent pass over npairs pairs for directions dir0..dir0+ndir-1;
+ * per-direction outputs land in c->pipe.out.  Optional polygon output with
+ * tangents (taxel Jacobian). */
+static int tangent_pass(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, int dir0, int ndir,
+                        double* poly_rec, double* poly_tan, int64_t* poly_key, int64_t poly_cap, Plan& plan,
+                        unsigned long long* npoly) {
+  int rc = stage_pairs(c, npairs, in, opts, plan);
+  if (rc) return rc;
+  const int nitems = (int)plan.items.size();
+  std::vector<double> sd;
+  seeds(c, in, npairs, sd);
+  DBuf dseed;
+  if ((rc = ensure(c, dseed, sizeof(double) * sd.size())) || (rc = ensure(c, c->d_poly_cnt, 64))) return rc;
+  rc = hip_check(c, hipMemcpyAsync(dseed.ptr, sd.data(), sd.size() * 8, hipMemcpyHostToDevice, c->stream), "upload");
+  Pipe& P = c->pipe;
+  for (int attempt = 0; attempt < 3 && !rc; ++attempt) {
+    rc = pipe_reserve(c, P, nitems, npairs, std::max<int64_t>(P.cap, 4096), ndir);
+    if (rc) break;
+    rc = hip_check(c, hipMemsetAsync(c->d_poly_cnt.ptr, 0, 64, c->stream), "memset");
+    if (rc) break;
+    QueryArgs a = pipe_args(P, (const EnvDesc*)c->d_envs.ptr, (const double*)c->d_state.ptr,
+                            (const Item*)c->d_items.ptr, nitems, false);
+    if (poly_rec) {
+      a.poly_rec = poly_rec, a.poly_tan = poly_tan, a.poly_key = poly_key, a.poly_cap = poly_cap;
+      a.poly_cnt = (unsigned long long*)c->d_poly_cnt.ptr;
+    }
+    pipe_enqueue(c, P, a, npairs, nitems, clip_grid_for(c, P.cap), ndir, (const double*)dseed.ptr,
+                 (const double*)dseed.ptr + npairs, false, dir0);
+    rc = hip_check(c, hipGetLastError(), "tangent pass");
+    unsigned long long ns = 0;
+    if (!rc) rc = hip_check(c, hipMemcpyAsync(&ns, P.surv_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream), "dl");
+    if (!rc && npoly) rc = hip_check(c, hipMemcpyAsync(npoly, c->d_poly_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream), "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    if (rc) break;
+    if ((int64_t)ns <= P.cap) break;
+    rc = pipe_reserve(c, P, nitems, npairs, (int64_t)ns + (int64_t)ns / 4 + 1024, ndir);
+  }
+  (void)hipFree(dseed.ptr);
+  if (rc) return rc;
+  int err = 0;
+  rc = hip_check(c, hipMemcpyAsync(&err, P.err.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (!rc && err) rc = set_err(c, HCS_ERR_LIMIT, "traversal stack overflow");
+  return rc;
 }
 
 }  // namespace hcs
@@ -110,55 +107,27 @@ extern "C" {
 
 int hcs_contact_jacobian(hcs_ctx* h, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts, double* J) {
   if (!h || npairs < 0 || (npairs > 0 && (!in || !J))) return HCS_ERR_ARG;
-  if (npairs == 0) return HCS_OK;
   Ctx* c = &h->c;
-  if (!kFusedTangentEnabled)
-    return set_err(c, HCS_ERR_STATE, "fused tangent query kernel is switched off (unexplained >180 s run on gfx950)");
+  if (!kTangentEnabled) return set_err(c, HCS_ERR_STATE, "differentiation is switched off in this build");
+  if (npairs == 0) return HCS_OK;
   Plan plan;
-  int rc = stage_pairs(c, npairs, in, opts, plan);
+  int rc = tangent_pass(c, npairs, in, opts, 0, kDirs, nullptr, nullptr, nullptr, 0, plan, nullptr);
   if (rc) return rc;
-  std::vector<double> invE, ddE;
-  seeds(c, in, npairs, invE, ddE);
-  DBuf dseed;
-  rc = ensure(c, dseed, sizeof(double) * 2 * npairs);
-  if (rc) return rc;
-  std::vector<double> sd(2 * (size_t)npairs);
-  std::copy(ddE.begin(), ddE.end(), sd.begin());
-  std::copy(invE.begin(), invE.end(), sd.begin() + npairs);
-  rc = hip_check(c, hipMemcpyAsync(dseed.ptr, sd.data(), sd.size() * 8, hipMemcpyHostToDevice, c->stream), "upload");
-  const int nitems = (int)plan.items.size();
-  std::vector<Partial> part((size_t)nitems * kDirs);
-  if (!rc && nitems > 0) {
-    QueryArgs a = tangent_args(c, plan);
-    launch_prep(c->stream, a.envs, a.state, (double*)c->d_xf.ptr, npairs);
-    launch_query_tangent(c->stream, a, kDirs, (const double*)dseed.ptr, (const double*)dseed.ptr + npairs,
-                         (Partial*)c->d_partials.ptr);
-    rc = hip_check(c, hipGetLastError(), "tangent query");
-    if (!rc)
-      rc = hip_check(c, hipMemcpyAsync(part.data(), c->d_partials.ptr, sizeof(Partial) * part.size(),
-                                       hipMemcpyDeviceToHost, c->stream),
-                     "download");
-  }
-  int err = 0;
-  if (!rc) rc = hip_check(c, hipMemcpyAsync(&err, c->d_err.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream), "dl");
+  std::vector<PairOut> o((size_t)npairs * kDirs);
+  rc = hip_check(c, hipMemcpyAsync(o.data(), c->pipe.out.ptr, sizeof(PairOut) * o.size(), hipMemcpyDeviceToHost,
+                                   c->stream),
+                 "download");
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
-  (void)hipFree(dseed.ptr);
   if (rc) return rc;
-  if (err) return set_err(c, HCS_ERR_LIMIT, "traversal stack overflow");
   for (int e = 0; e < npairs; ++e) {
-    const EnvDesc& E = plan.envs[e];
     double* Je = J + (size_t)e * 12 * kDirs;
     for (int d = 0; d < kDirs; ++d) {
-      double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
-      for (int i = E.item_first; i < E.item_first + E.item_count; ++i) {
-        const Partial& q = part[(size_t)d * nitems + i];
-        for (int k = 0; k < 3; ++k) F[k] += q.F[k], tA[k] += q.tA[k], tB[k] += q.tB[k];
-      }
+      const PairOut& r = o[(size_t)d * npairs + e];
       for (int k = 0; k < 3; ++k) {
-        Je[(0 + k) * kDirs + d] = F[k];
-        Je[(3 + k) * kDirs + d] = tA[k];
-        Je[(6 + k) * kDirs + d] = -F[k];
-        Je[(9 + k) * kDirs + d] = tB[k];
+        Je[(0 + k) * kDirs + d] = r.F_A[k];
+        Je[(3 + k) * kDirs + d] = r.tau_A[k];
+        Je[(6 + k) * kDirs + d] = r.F_B[k];
+        Je[(9 + k) * kDirs + d] = r.tau_B[k];
       }
     }
   }
@@ -184,43 +153,31 @@ int hcs_taxel_jacobian(hcs_ctx* h, const hcs_pair_in* pair, const hcs_step_opts*
                        double* Jt) {
   if (!h || !pair || !G || !Jt) return HCS_ERR_ARG;
   Ctx* c = &h->c;
-  if (!kFusedTangentEnabled)
-    return set_err(c, HCS_ERR_STATE, "fused tangent query kernel is switched off (unexplained >180 s run on gfx950)");
+  if (!kTangentEnabled) return set_err(c, HCS_ERR_STATE, "differentiation is switched off in this build");
   TaxelGridDev g;
   if (taxel_grid_dev(*G, g) != HCS_OK) return set_err(c, HCS_ERR_ARG, "invalid taxel grid");
   const int ncell = g.cx * g.cy;
-  Plan plan;
   hcs_step_opts o = opts ? *opts : hcs_step_opts{0, 1, 0.001, 0, 0};
-  int rc = stage_pairs(c, 1, pair, &o, plan);
-  if (rc) return rc;
-  std::vector<double> invE, ddE;
-  seeds(c, pair, 1, invE, ddE);
-  const int nitems = (int)plan.items.size();
-  /* polygon capacity from a primal count run */
+  /* polygon capacity from a primal run */
   hcs_pair_out po;
   hcs_step_opts oc = o;
-  oc.keep_polygons = 1;
-  rc = hcs_contact_step(h, 1, pair, &oc, &po, nullptr);
+  oc.keep_polygons = 0;
+  int rc = hcs_contact_step(h, 1, pair, &oc, &po, nullptr);
   if (rc) return rc;
   const int64_t cap = std::max<int64_t>(1, po.n_polygons);
-  rc = stage_pairs(c, 1, pair, &o, plan);
-  if (rc) return rc;
-  DBuf prec, ptan, pkey, dseed, dXd, dmask, dout;
+  DBuf prec, ptan, pkey, dXd, dmask, dout;
   auto cleanup = [&]() {
-    for (DBuf* b : {&prec, &ptan, &pkey, &dseed, &dXd, &dmask, &dout})
+    for (DBuf* b : {&prec, &ptan, &pkey, &dXd, &dmask, &dout})
       if (b->ptr) (void)hipFree(b->ptr);
   };
   if ((rc = ensure(c, prec, sizeof(double) * HCS_PREC * cap)) || (rc = ensure(c, ptan, sizeof(double) * HCS_PREC * cap)) ||
-      (rc = ensure(c, pkey, sizeof(int64_t) * 3 * cap)) || (rc = ensure(c, dseed, sizeof(double) * 2)) ||
-      (rc = ensure(c, dXd, sizeof(double) * 12)) || (rc = ensure(c, dmask, 8)) ||
-      (rc = ensure(c, dout, sizeof(double) * ncell))) {
+      (rc = ensure(c, pkey, sizeof(int64_t) * 3 * cap)) || (rc = ensure(c, dXd, sizeof(double) * 12)) ||
+      (rc = ensure(c, dmask, 8)) || (rc = ensure(c, dout, sizeof(double) * ncell))) {
     cleanup();
     return rc;
   }
-  const double sd[2] = {ddE[0], invE[0]};
   const unsigned char one = 1;
-  rc = hip_check(c, hipMemcpyAsync(dseed.ptr, sd, sizeof sd, hipMemcpyHostToDevice, c->stream), "upload");
-  if (!rc) rc = hip_check(c, hipMemcpyAsync(dmask.ptr, &one, 1, hipMemcpyHostToDevice, c->stream), "upload");
+  rc = hip_check(c, hipMemcpyAsync(dmask.ptr, &one, 1, hipMemcpyHostToDevice, c->stream), "upload");
   const double* R = pair->X_WB;
   for (int d = 0; d < kDirs && !rc; ++d) {
     /* tangent of the sensor pose (the sensor rides on geometry B) */
@@ -238,21 +195,10 @@ int hcs_taxel_jacobian(hcs_ctx* h, const hcs_pair_in* pair, const hcs_step_opts*
       Xd[9 + (d - 3)] = 1.0;
     }
     rc = hip_check(c, hipMemcpyAsync(dXd.ptr, Xd, sizeof Xd, hipMemcpyHostToDevice, c->stream), "upload");
-    if (!rc) rc = hip_check(c, hipMemsetAsync(c->d_poly_cnt.ptr, 0, 64, c->stream), "memset");
     if (rc) break;
-    QueryArgs a = tangent_args(c, plan);
-    a.poly_rec = (double*)prec.ptr;
-    a.poly_key = (int64_t*)pkey.ptr;
-    a.poly_cnt = (unsigned long long*)c->d_poly_cnt.ptr;
-    a.poly_cap = cap;
-    a.poly_tan = (double*)ptan.ptr;
-    launch_prep(c->stream, a.envs, a.state, (double*)c->d_xf.ptr, 1);
-    launch_query_tangent_dir(c->stream, a, d, (const double*)dseed.ptr, (const double*)dseed.ptr + 1,
-                             (Partial*)c->d_partials.ptr);
-    rc = hip_check(c, hipGetLastError(), "tangent query");
+    Plan plan;
     unsigned long long np = 0;
-    if (!rc) rc = hip_check(c, hipMemcpyAsync(&np, c->d_poly_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream), "dl");
-    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    rc = tangent_pass(c, 1, pair, &o, d, 1, (double*)prec.ptr, (double*)ptan.ptr, (int64_t*)pkey.ptr, cap, plan, &np);
     if (rc) break;
     const int64_t n = std::min<int64_t>((int64_t)np, cap);
     rc = run_taxels(c, g, (const unsigned char*)dmask.ptr, (const double*)prec.ptr, (const double*)ptan.ptr,
@@ -264,7 +210,6 @@ int hcs_taxel_jacobian(hcs_ctx* h, const hcs_pair_in* pair, const hcs_step_opts*
     if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
     for (int k = 0; k < ncell && !rc; ++k) Jt[(size_t)k * kDirs + d] = col[k];
   }
-  (void)nitems;
   cleanup();
   return rc;
 }

@@ -1,14 +1,16 @@
 /*
  * hcs_batch.hip -- batched form: thousands of independent pairs stepped by
- * one persistent work-queue launch, the step captured once in a hipGraph
- * and replayed.  No reference counterpart (the reference steps one pair at a
- * time inside mj_collision, mujoco_contact_surfaces_plugin.cpp:92-95).
+ * one graph replay.  No reference counterpart (the reference steps one pair
+ * at a time inside mj_collision, mujoco_contact_surfaces_plugin.cpp:92-95).
  *
- * Step = [reset queue counter (fill kernel, no memset node), per-pair pose
- * composition, persistent query/clip/force kernel, per-pair reduction].
- * The autotuner times the launch shapes (persistent blocks per CU, queries
- * per lane, the tet-query threshold for tiny rigid meshes) on the current
- * state and keeps the fastest; results never depend on the shape.
+ * Step = [reset counters (fill kernels, no memset nodes), per-pair pose
+ * composition, persistent candidate kernel over a work queue, scan of the
+ * per-item survivor counts, clip/force kernel, per-pair reduction].  The
+ * survivor buffer grows when a step overflows it (checked in
+ * hcs_batch_get, which then re-runs the step).  The autotuner times the
+ * launch shapes (persistent blocks per CU, the tet-query threshold for small
+ * rigid meshes) on the current state and keeps the fastest; results never
+ * depend on the shape.
  */
 #include <hip/hip_runtime.h>
 
@@ -27,7 +29,8 @@ struct hcs_batch {
   hcs_step_opts opts{};
   Plan plan;
   int qpl = 1, blocks_per_cu = 4, tet_max = 64;
-  DBuf envs, state, xf, items, partials, out, err, counter;
+  DBuf envs, state, items;
+  Pipe pipe;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -46,27 +49,14 @@ static int grid_of(const hcs_batch* b) {
 }
 
 static QueryArgs batch_args(hcs_batch* b) {
-  QueryArgs a{};
-  a.envs = (const EnvDesc*)b->envs.ptr;
-  a.state = (const double*)b->state.ptr;
-  a.xf = (const double*)b->xf.ptr;
-  a.items = (const Item*)b->items.ptr;
-  a.nitems = (int)b->plan.items.size();
-  a.queries_per_lane = b->qpl;
-  a.counter = (unsigned int*)b->counter.ptr;
-  a.partials = (Partial*)b->partials.ptr;
-  a.error = (int*)b->err.ptr;
-  return a;
+  return pipe_args(b->pipe, (const EnvDesc*)b->envs.ptr, (const double*)b->state.ptr, (const Item*)b->items.ptr,
+                   (int)b->plan.items.size(), true);
 }
 
 /* Enqueues one step on the context stream. */
-static void batch_enqueue(hcs_batch* b) {
-  hipStream_t s = b->c->stream;
-  QueryArgs a = batch_args(b);
-  launch_fill_u32(s, a.counter, 0u, 1);
-  launch_prep(s, a.envs, a.state, (double*)b->xf.ptr, b->npairs);
-  launch_query(s, a, grid_of(b));
-  launch_reduce(s, a.envs, a.partials, a.error, (PairOut*)b->out.ptr, b->npairs);
+static void batch_enqueue(hcs_batch* b, const QueryArgs& a) {
+  pipe_enqueue(b->c, b->pipe, a, b->npairs, grid_of(b), clip_grid_for(b->c, b->pipe.cap), 0, nullptr, nullptr,
+               false, 0);
 }
 
 /* (Re)builds the plan for the current shape and uploads it. */
@@ -78,9 +68,8 @@ static int batch_plan(hcs_batch* b) {
   const size_t nitems = std::max<size_t>(1, b->plan.items.size());
   if ((rc = ensure(c, b->envs, sizeof(EnvDesc) * b->npairs)) ||
       (rc = ensure(c, b->state, sizeof(double) * HCS_STATE * b->npairs)) ||
-      (rc = ensure(c, b->xf, sizeof(double) * 12 * b->npairs)) || (rc = ensure(c, b->items, sizeof(Item) * nitems)) ||
-      (rc = ensure(c, b->partials, sizeof(Partial) * nitems)) || (rc = ensure(c, b->out, sizeof(PairOut) * b->npairs)) ||
-      (rc = ensure(c, b->err, 64)) || (rc = ensure(c, b->counter, 64)))
+      (rc = ensure(c, b->items, sizeof(Item) * nitems)) ||
+      (rc = pipe_reserve(c, b->pipe, (int)b->plan.items.size(), b->npairs, std::max<int64_t>(b->pipe.cap, 65536), 0)))
     return rc;
   rc = hip_check(c, hipMemcpyAsync(b->envs.ptr, b->plan.envs.data(), sizeof(EnvDesc) * b->npairs,
                                    hipMemcpyHostToDevice, c->stream),
@@ -91,8 +80,6 @@ static int batch_plan(hcs_batch* b) {
                                      hipMemcpyHostToDevice, c->stream),
                    "upload items");
   if (rc) return rc;
-  rc = hip_check(c, hipMemsetAsync(b->err.ptr, 0, 64, c->stream), "memset");
-  if (rc) return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
@@ -101,10 +88,28 @@ static int batch_capture(hcs_batch* b) {
   batch_free_graph(b);
   int rc = hip_check(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
   if (rc) return rc;
-  batch_enqueue(b);
+  batch_enqueue(b, batch_args(b));
   rc = hip_check(c, hipStreamEndCapture(c->stream, &b->graph), "EndCapture");
   if (rc) return rc;
   return hip_check(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "GraphInstantiate");
+}
+
+/* Survivor count of the last step; grows the buffer and re-runs the step
+ * when it overflowed.  Returns HCS_OK or an error. */
+static int batch_settle(hcs_batch* b) {
+  Ctx* c = b->c;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    unsigned long long n = 0;
+    int rc = hip_check(c, hipMemcpyAsync(&n, b->pipe.surv_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream), "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
+    if ((int64_t)n <= b->pipe.cap) return HCS_OK;
+    rc = pipe_reserve(c, b->pipe, (int)b->plan.items.size(), b->npairs, (int64_t)n + (int64_t)n / 4 + 4096, 0);
+    if (!rc) rc = batch_capture(b);
+    if (!rc) rc = hcs_batch_step(b);
+    if (rc) return rc;
+  }
+  return set_err(c, HCS_ERR_CAPACITY, "survivor buffer did not settle");
 }
 
 extern "C" {
@@ -148,8 +153,9 @@ int hcs_batch_destroy(hcs_batch* b) {
   if (!b) return HCS_ERR_ARG;
   (void)hipStreamSynchronize(b->c->stream);
   batch_free_graph(b);
-  for (DBuf* d : {&b->envs, &b->state, &b->xf, &b->items, &b->partials, &b->out, &b->err, &b->counter})
+  for (DBuf* d : {&b->envs, &b->state, &b->items})
     if (d->ptr) (void)hipFree(d->ptr);
+  pipe_free(b->pipe);
   delete b;
   return HCS_OK;
 }
@@ -168,20 +174,22 @@ int hcs_batch_step(hcs_batch* b) {
   if (!b) return HCS_ERR_ARG;
   Ctx* c = b->c;
   if (b->exec) return hip_check(c, hipGraphLaunch(b->exec, c->stream), "hipGraphLaunch");
-  batch_enqueue(b);
+  batch_enqueue(b, batch_args(b));
   return hip_check(c, hipGetLastError(), "batch step");
 }
 
 int hcs_batch_get(hcs_batch* b, hcs_pair_out* out) {
   if (!b || !out) return HCS_ERR_ARG;
   Ctx* c = b->c;
-  int rc = hip_check(c, hipMemcpyAsync(out, b->out.ptr, sizeof(PairOut) * b->npairs, hipMemcpyDeviceToHost, c->stream),
+  int rc = batch_settle(b);
+  if (rc) return rc;
+  rc = hip_check(c, hipMemcpyAsync(out, b->pipe.out.ptr, sizeof(PairOut) * b->npairs, hipMemcpyDeviceToHost, c->stream),
                      "download");
   if (rc) return rc;
   rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
   if (rc) return rc;
   for (int e = 0; e < b->npairs; ++e)
-    if (out[e].status) return set_err(c, (int)out[e].status, "traversal stack overflow");
+    if (out[e].status) return set_err(c, (int)out[e].status, "traversal stack overflow or survivor overflow");
   return HCS_OK;
 }
 
@@ -216,6 +224,7 @@ static int time_steps(hcs_batch* b, int reps, double* ms) {
   rc = hip_check(c, hipEventCreate(&e1), "event");
   if (rc) return rc;
   rc = hcs_batch_step(b); /* warm-up */
+  if (!rc) rc = batch_settle(b);
   if (!rc) rc = hip_check(c, hipEventRecord(e0, c->stream), "record");
   for (int r = 0; r < reps && !rc; ++r) rc = hcs_batch_step(b);
   if (!rc) rc = hip_check(c, hipEventRecord(e1, c->stream), "record");
@@ -231,8 +240,8 @@ static int time_steps(hcs_batch* b, int reps, double* ms) {
 int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
   if (!b || reps <= 0) return HCS_ERR_ARG;
   const int bpc[] = {1, 2, 4, 8};
-  const int qpls[] = {1, 2};
-  const int tets[] = {16, 64, 256};
+  const int qpls[] = {1};
+  const int tets[] = {64, 256, 1024};
   double best = 1e300;
   int bb = b->blocks_per_cu, bq = b->qpl, bt = b->tet_max;
   for (int t : tets)
@@ -266,9 +275,7 @@ int hcs_batch_item_stats(hcs_batch* b, int64_t* out, int64_t capacity, int64_t* 
   if (rc) return rc;
   QueryArgs a = batch_args(b);
   a.item_stats = (unsigned long long*)d.ptr;
-  launch_fill_u32(c->stream, a.counter, 0u, 1);
-  launch_prep(c->stream, a.envs, a.state, (double*)b->xf.ptr, b->npairs);
-  launch_query(c->stream, a, grid_of(b));
+  batch_enqueue(b, a);
   std::vector<unsigned long long> h(4 * (size_t)n);
   rc = hip_check(c, hipGetLastError(), "stats step");
   if (!rc) rc = hip_check(c, hipMemcpyAsync(h.data(), d.ptr, h.size() * 8, hipMemcpyDeviceToHost, c->stream), "dl");
@@ -295,23 +302,27 @@ int hcs_batch_kernel_times(hcs_batch* b, int32_t reps, double* ms) {
     if (rc) return rc;
   }
   double acc[3] = {0, 0, 0};
-  int rc = HCS_OK;
+  int rc = batch_settle(b);
   for (int r = 0; r < reps && !rc; ++r) {
     QueryArgs a = batch_args(b);
-    launch_fill_u32(s, a.counter, 0u, 1);
+    Pipe& P = b->pipe;
+    launch_fill_u32(s, (unsigned*)P.counter.ptr, 0u, 1);
+    launch_fill_u32(s, (unsigned*)P.surv_cnt.ptr, 0u, 2);
+    launch_fill_u32(s, (unsigned*)P.err.ptr, 0u, 1);
+    launch_fill_u32(s, (unsigned*)P.nsurv.ptr + a.nitems, 0u, 1);
+    launch_prep(s, a.envs, a.state, (double*)P.xf.ptr, b->npairs);
     (void)hipEventRecord(ev[0], s);
-    launch_prep(s, a.envs, a.state, (double*)b->xf.ptr, b->npairs);
+    launch_cand(s, a, grid_of(b));
     (void)hipEventRecord(ev[1], s);
-    launch_query(s, a, grid_of(b));
-    (void)hipEventRecord(ev[2], s);
-    launch_reduce(s, a.envs, a.partials, a.error, (PairOut*)b->out.ptr, b->npairs);
+    launch_clip_reduce(c, P, a, b->npairs, clip_grid_for(c, P.cap), ev[2], ev[4]);
     (void)hipEventRecord(ev[3], s);
     rc = hip_check(c, hipEventSynchronize(ev[3]), "sync");
-    float t;
-    for (int k = 0; k < 3 && !rc; ++k) {
-      rc = hip_check(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]), "elapsed");
-      acc[k] += t;
-    }
+    float t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    if (!rc) rc = hip_check(c, hipEventElapsedTime(&t0, ev[0], ev[1]), "elapsed");
+    if (!rc) rc = hip_check(c, hipEventElapsedTime(&t1, ev[2], ev[4]), "elapsed");
+    if (!rc) rc = hip_check(c, hipEventElapsedTime(&t2, ev[1], ev[2]), "elapsed");
+    if (!rc) rc = hip_check(c, hipEventElapsedTime(&t3, ev[4], ev[3]), "elapsed");
+    acc[0] += t0, acc[1] += t1, acc[2] += t2 + t3;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   for (int k = 0; k < 3; ++k) ms[k] = acc[k] / reps;

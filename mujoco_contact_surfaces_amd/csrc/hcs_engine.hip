@@ -279,83 +279,181 @@ __global__ void k_prep(const EnvDesc* __restrict__ envs, const double* __restric
   for (int i = 0; i < 12; ++i) xf[12 * (size_t)e + i] = X[i];
 }
 
-template <class T>
-__global__ __launch_bounds__(64) void k_query(QueryArgs A, int dir_base, const double* dstar_dE, const double* invE,
-                                              Partial* tpart) {
-  __shared__ WaveLds<T> L;
-  const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
-                  A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan};
-  const OutPtrs* op = (A.face_rec || A.poly_rec) ? &o : nullptr;
-  const int dir = dir_base < 0 ? -1 : dir_base + (int)blockIdx.y;
-  Partial* part = dir < 0 ? A.partials : tpart + (size_t)(dir - dir_base) * A.nitems;
-  const int it = blockIdx.x;
-  if (it < A.nitems) run_item<T>(L, A, it, dir, dstar_dE, invE, part + it, op);
-}
-
-/* Persistent form of k_query for the batched step: every wave takes items
- * from a global counter until none is left (the counter is reset by a fill
- * kernel in the same graph). */
-__global__ __launch_bounds__(64) void k_step_persistent(QueryArgs A) {
-  __shared__ WaveLds<double> L;
+/* Candidate generation: one wave per item; persistent when a work-queue
+ * counter is given (the counter is reset by a fill kernel in the same
+ * graph). */
+__global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
+  __shared__ CandLds L;
+  if (A.counter == nullptr) {
+    if ((int)blockIdx.x < A.nitems) run_cand(L, A, blockIdx.x);
+    return;
+  }
   for (;;) {
     unsigned v = 0;
     if (threadIdx.x == 0) v = atomicAdd(A.counter, 1u);
     const int it = (int)__shfl(v, 0, 64);
     if (it >= A.nitems) break;
-    run_item<double>(L, A, it, -1, nullptr, nullptr, A.partials + it, nullptr);
+    run_cand(L, A, it);
   }
 }
 
-/* Per pair: partials of its items summed in item order. */
-__global__ void k_reduce(const EnvDesc* __restrict__ envs, const Partial* __restrict__ partials,
-                         const int* __restrict__ err, PairOut* __restrict__ out, int n) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+/* Clip, faces and force of every survivor (grid-stride); grid.y selects the
+ * tangent direction in the Dual instantiation. */
+template <class T>
+__global__ __launch_bounds__(256) void k_clip(QueryArgs A, ClipArgs C, int dir_base, Contrib* out,
+                                              int64_t out_stride) {
+  const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
+                  A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan};
+  const OutPtrs* op = (A.face_rec || A.poly_rec) ? &o : nullptr;
+  const int dir = dir_base < 0 ? -1 : dir_base + (int)blockIdx.y;
+  Contrib* dst = out + (dir < 0 ? 0 : (int64_t)(dir - dir_base) * out_stride);
+  const unsigned long long cnt = *A.surv_cnt;
+  const int64_t n = (int64_t)cnt < A.surv_cap ? (int64_t)cnt : A.surv_cap;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    clip_survivor<T>(A, C, i, dir, dst, op);
+}
+
+/* Per pair (one wave, grid.y = direction): its survivors' contributions
+ * summed lane-strided in order, then a fixed butterfly; candidates summed
+ * over its items. */
+__global__ __launch_bounds__(64) void k_env_reduce(const EnvDesc* __restrict__ envs, const unsigned* __restrict__ off,
+                                                   const int* __restrict__ item_ncand,
+                                                   const unsigned long long* __restrict__ surv_cnt, int64_t surv_cap,
+                                                   const Contrib* __restrict__ contrib, int64_t stride,
+                                                   const int* __restrict__ err, PairOut* __restrict__ out, int n) {
+  const int e = blockIdx.x, d = blockIdx.y, lane = threadIdx.x;
   const int first = envs[e].item_first, cnt = envs[e].item_count;
+  /* on survivor-list overflow the offsets run past the contribution slots:
+   * sum nothing, the status below reports the overflow */
+  const bool over = (int64_t)*surv_cnt > surv_cap;
+  const unsigned lo = off[first], hi = over ? lo : off[first + cnt];
+  const Contrib* cb = contrib + (int64_t)d * stride;
   double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
-  int64_t nc = 0, np = 0, nf = 0;
-  for (int i = first; i < first + cnt; ++i) {
-    const Partial& q = partials[i];
+  int64_t np = 0, nf = 0, nc = 0;
+  for (unsigned i = lo + lane; i < hi; i += 64) {
+    const Contrib& q = cb[i];
 #pragma unroll
     for (int c = 0; c < 3; ++c) F[c] = F[c] + q.F[c], tA[c] = tA[c] + q.tA[c], tB[c] = tB[c] + q.tB[c];
-    nc += q.nc, np += q.np, nf += q.nf;
+    np += q.np, nf += q.nf;
   }
-  PairOut r;
+  for (int i = first + lane; i < first + cnt; i += 64) nc += item_ncand[i];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) r.F_A[c] = F[c], r.tau_A[c] = tA[c], r.F_B[c] = -F[c], r.tau_B[c] = tB[c];
-  r.n_candidates = nc, r.n_polygons = np, r.n_faces = nf;
-  r.status = (err && *err) ? HCS_ERR_LIMIT : 0;
-  out[e] = r;
+  for (int c = 0; c < 3; ++c) F[c] = wave_sum(F[c]), tA[c] = wave_sum(tA[c]), tB[c] = wave_sum(tB[c]);
+  np = wave_sum_i(np), nf = wave_sum_i(nf), nc = wave_sum_i(nc);
+  if (lane == 0) {
+    PairOut r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r.F_A[c] = F[c], r.tau_A[c] = tA[c], r.F_B[c] = -F[c], r.tau_B[c] = tB[c];
+    r.n_candidates = nc, r.n_polygons = np, r.n_faces = nf;
+    r.status = (err && *err) ? HCS_ERR_LIMIT : (over ? HCS_ERR_CAPACITY : 0);
+    out[(int64_t)d * n + e] = r;
+  }
 }
 
 void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double* xf, int n) {
   if (n > 0) hipLaunchKernelGGL(k_prep, dim3((n + 127) / 128), dim3(128), 0, s, envs, state, xf, n);
 }
-void launch_query(hipStream_t s, const QueryArgs& a, int grid) {
-  if (grid <= 0) return;
-  if (a.counter)
-    hipLaunchKernelGGL(k_step_persistent, dim3(grid), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_query<double>), dim3(grid), dim3(64), 0, s, a, -1, (const double*)nullptr,
-                       (const double*)nullptr, (Partial*)nullptr);
-}
-void launch_query_tangent(hipStream_t s, const QueryArgs& a, int ndir, const double* dstar_dE, const double* invE,
-                          Partial* tpart) {
-  if (a.nitems > 0)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_query<Dual>), dim3(a.nitems, ndir), dim3(64), 0, s, a, 0, dstar_dE, invE,
-                       tpart);
-}
-void launch_query_tangent_dir(hipStream_t s, const QueryArgs& a, int dir, const double* dstar_dE,
-                              const double* invE, Partial* tpart) {
-  if (a.nitems > 0)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_query<Dual>), dim3(a.nitems, 1), dim3(64), 0, s, a, dir, dstar_dE, invE,
-                       tpart);
-}
-void launch_reduce(hipStream_t s, const EnvDesc* envs, const Partial* partials, const int* err, PairOut* out, int n) {
-  if (n > 0) hipLaunchKernelGGL(k_reduce, dim3((n + 127) / 128), dim3(128), 0, s, envs, partials, err, out, n);
-}
 void launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, int n) {
   if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+}
+
+/* Pipeline buffers (grow-only) and the enqueue of one pass. */
+int pipe_reserve(Ctx* c, Pipe& p, int nitems, int npairs, int64_t cap, int ndir) {
+  int rc;
+  const size_t ni = (size_t)std::max(1, nitems);
+  if ((rc = ensure(c, p.counter, 64)) || (rc = ensure(c, p.surv_cnt, 64)) || (rc = ensure(c, p.err, 64)) ||
+      (rc = ensure(c, p.xf, sizeof(double) * 12 * std::max(1, npairs))) ||
+      (rc = ensure(c, p.ncand, sizeof(int) * ni)) || (rc = ensure(c, p.nsurv, sizeof(unsigned) * (ni + 1))) ||
+      (rc = ensure(c, p.off, sizeof(unsigned) * (ni + 1))) ||
+      (rc = ensure(c, p.out, sizeof(PairOut) * std::max(1, npairs) * std::max(1, ndir))))
+    return rc;
+  cap = std::max<int64_t>(cap, 4096);
+  if ((rc = ensure(c, p.surv, sizeof(int4) * cap)) ||
+      (rc = ensure(c, p.contrib, sizeof(Contrib) * cap * std::max(1, ndir))))
+    return rc;
+  p.cap = cap;
+  size_t bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr, (int)ni + 1,
+                                   c->stream);
+  if ((rc = ensure(c, p.scan_tmp, std::max<size_t>(bytes, 8)))) return rc;
+  p.scan_bytes = bytes;
+  return HCS_OK;
+}
+
+QueryArgs pipe_args(const Pipe& p, const EnvDesc* envs, const double* state, const Item* items, int nitems,
+                    bool persistent) {
+  QueryArgs a{};
+  a.envs = envs;
+  a.state = state;
+  a.xf = (const double*)p.xf.ptr;
+  a.items = items;
+  a.nitems = nitems;
+  a.queries_per_lane = 1;
+  a.counter = persistent ? (unsigned*)p.counter.ptr : nullptr;
+  a.error = (int*)p.err.ptr;
+  a.surv = (int4*)p.surv.ptr;
+  a.surv_cnt = (unsigned long long*)p.surv_cnt.ptr;
+  a.surv_cap = p.cap;
+  a.item_ncand = (int*)p.ncand.ptr;
+  a.item_nsurv = (unsigned*)p.nsurv.ptr;
+  return a;
+}
+
+void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid, int clip_grid, int ndir,
+                  const double* dstar_dE, const double* invE, bool cand_only, int dir0) {
+  hipStream_t s = c->stream;
+  launch_fill_u32(s, (unsigned*)p.counter.ptr, 0u, 1);
+  launch_fill_u32(s, (unsigned*)p.surv_cnt.ptr, 0u, 2);
+  launch_fill_u32(s, (unsigned*)p.err.ptr, 0u, 1);
+  launch_fill_u32(s, (unsigned*)p.nsurv.ptr + a.nitems, 0u, 1);
+  launch_prep(s, a.envs, a.state, (double*)p.xf.ptr, npairs);
+  if (a.nitems > 0 && cand_grid > 0) hipLaunchKernelGGL(k_cand, dim3(cand_grid), dim3(64), 0, s, a);
+  if (cand_only) return;
+  size_t bytes = p.scan_bytes;
+  hipcub::DeviceScan::ExclusiveSum(p.scan_tmp.ptr, bytes, (const unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr,
+                                   a.nitems + 1, s);
+  ClipArgs C{(const unsigned*)p.off.ptr, dstar_dE, invE, p.cap};
+  const int g = std::max(1, clip_grid);
+  if (ndir <= 0)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double>), dim3(g), dim3(256), 0, s, a, C, -1, (Contrib*)p.contrib.ptr,
+                       (int64_t)0);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<Dual>), dim3(g, ndir), dim3(256), 0, s, a, C, dir0,
+                       (Contrib*)p.contrib.ptr, (int64_t)p.cap);
+  if (npairs > 0)
+    hipLaunchKernelGGL(k_env_reduce, dim3(npairs, std::max(1, ndir)), dim3(64), 0, s, a.envs,
+                       (const unsigned*)p.off.ptr, (const int*)p.ncand.ptr, (const unsigned long long*)p.surv_cnt.ptr,
+                       p.cap, (const Contrib*)p.contrib.ptr, (int64_t)p.cap, (const int*)p.err.ptr,
+                       (PairOut*)p.out.ptr, npairs);
+}
+
+void launch_cand(hipStream_t s, const QueryArgs& a, int grid) {
+  if (a.nitems > 0 && grid > 0) hipLaunchKernelGGL(k_cand, dim3(grid), dim3(64), 0, s, a);
+}
+
+/* scan, event e_clip, k_clip, event e_reduce, k_env_reduce (forward). */
+void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, hipEvent_t e_clip,
+                        hipEvent_t e_reduce) {
+  hipStream_t s = c->stream;
+  size_t bytes = p.scan_bytes;
+  hipcub::DeviceScan::ExclusiveSum(p.scan_tmp.ptr, bytes, (const unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr,
+                                   a.nitems + 1, s);
+  (void)hipEventRecord(e_clip, s);
+  ClipArgs C{(const unsigned*)p.off.ptr, nullptr, nullptr, p.cap};
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double>), dim3(std::max(1, clip_grid)), dim3(256), 0, s, a, C, -1,
+                     (Contrib*)p.contrib.ptr, (int64_t)0);
+  (void)hipEventRecord(e_reduce, s);
+  if (npairs > 0)
+    hipLaunchKernelGGL(k_env_reduce, dim3(npairs, 1), dim3(64), 0, s, a.envs, (const unsigned*)p.off.ptr,
+                       (const int*)p.ncand.ptr, (const unsigned long long*)p.surv_cnt.ptr, p.cap,
+                       (const Contrib*)p.contrib.ptr, (int64_t)p.cap, (const int*)p.err.ptr, (PairOut*)p.out.ptr,
+                       npairs);
+}
+
+void pipe_free(Pipe& p) {
+  for (DBuf* b : {&p.counter, &p.surv, &p.surv_cnt, &p.ncand, &p.nsurv, &p.off, &p.scan_tmp, &p.contrib, &p.out, &p.err,
+                  &p.xf})
+    if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
 }
 
 /* ===================================================================
@@ -562,8 +660,8 @@ int hcs_destroy(hcs_ctx* h) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto& m : c->meshes) free_mesh(m);
-  for (DBuf* b : {&c->d_envs, &c->d_state, &c->d_xf, &c->d_items, &c->d_partials, &c->d_out, &c->d_err,
-                  &c->d_face_rec, &c->d_face_key, &c->d_face_cnt, &c->d_poly_rec, &c->d_poly_key, &c->d_poly_cnt,
+  pipe_free(c->pipe);
+  for (DBuf* b : {&c->d_envs, &c->d_state, &c->d_items, &c->d_face_rec, &c->d_face_key, &c->d_face_cnt, &c->d_poly_rec, &c->d_poly_key, &c->d_poly_cnt,
                   &c->d_cand, &c->d_cand_cnt, &c->d_tx_cnt, &c->d_tx_off, &c->d_tx_list, &c->d_tx_out,
                   &c->d_tx_grid})
     if (b->ptr) (void)hipFree(b->ptr);
@@ -669,32 +767,51 @@ int hcs_mesh_tet_records(hcs_ctx* h, int32_t handle, double* out) {
 
 namespace hcs {
 
-/* Shared body of hcs_contact_step / hcs_candidates: plan, upload, query,
- * reduce.  keys: candidate-key mode. */
-static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, hcs_pair_out* out,
-                     hcs_faces* faces, int64_t* keys, int64_t key_cap, int64_t* key_count) {
+/* Uploads plan and state of npairs pairs into the context scratch. */
+int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan) {
   HCHK(c, hipSetDevice(c->device));
   std::vector<int32_t> geoms(2 * (size_t)npairs);
   std::vector<double> state((size_t)HCS_STATE * npairs);
   for (int e = 0; e < npairs; ++e) {
     geoms[2 * e] = in[e].geom_a, geoms[2 * e + 1] = in[e].geom_b;
-    double* s = &state[(size_t)HCS_STATE * e];
-    std::memcpy(s, in[e].X_WA, 12 * sizeof(double));
-    std::memcpy(s + 12, in[e].X_WB, 12 * sizeof(double));
-    std::memcpy(s + 24, in[e].V_WA, 6 * sizeof(double));
-    std::memcpy(s + 30, in[e].V_WB, 6 * sizeof(double));
+    double* st = &state[(size_t)HCS_STATE * e];
+    std::memcpy(st, in[e].X_WA, 12 * sizeof(double));
+    std::memcpy(st + 12, in[e].X_WB, 12 * sizeof(double));
+    std::memcpy(st + 24, in[e].V_WA, 6 * sizeof(double));
+    std::memcpy(st + 30, in[e].V_WB, 6 * sizeof(double));
   }
-  Plan plan;
   int rc = build_plan(c, npairs, geoms.data(), opts, 1, c->tet_mode_max_tris, plan);
-  if (rc != HCS_OK) return rc;
+  if (rc) return rc;
   const int nitems = (int)plan.items.size();
   if ((rc = ensure(c, c->d_envs, sizeof(EnvDesc) * npairs)) || (rc = ensure(c, c->d_state, state.size() * 8)) ||
-      (rc = ensure(c, c->d_xf, 12 * sizeof(double) * npairs)) ||
-      (rc = ensure(c, c->d_items, sizeof(Item) * std::max(1, nitems))) ||
-      (rc = ensure(c, c->d_partials, sizeof(Partial) * std::max(1, nitems))) ||
-      (rc = ensure(c, c->d_out, sizeof(PairOut) * npairs)) || (rc = ensure(c, c->d_err, 64)) ||
-      (rc = ensure(c, c->d_face_cnt, 64)) || (rc = ensure(c, c->d_poly_cnt, 64)) ||
-      (rc = ensure(c, c->d_cand_cnt, 64)))
+      (rc = ensure(c, c->d_items, sizeof(Item) * std::max(1, nitems))))
+    return rc;
+  HCHK(c, hipMemcpyAsync(c->d_envs.ptr, plan.envs.data(), sizeof(EnvDesc) * npairs, hipMemcpyHostToDevice,
+                         c->stream));
+  HCHK(c, hipMemcpyAsync(c->d_state.ptr, state.data(), state.size() * 8, hipMemcpyHostToDevice, c->stream));
+  if (nitems)
+    HCHK(c, hipMemcpyAsync(c->d_items.ptr, plan.items.data(), sizeof(Item) * nitems, hipMemcpyHostToDevice,
+                           c->stream));
+  c->last_geoms = geoms;
+  return HCS_OK;
+}
+
+int clip_grid_for(const Ctx* c, int64_t cap) {
+  const int64_t blocks = (cap + 255) / 256;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, (int64_t)c->num_cu * 16));
+}
+
+/* Shared body of hcs_contact_step / hcs_candidates: one pass of the
+ * pipeline over npairs pairs (keys: candidate-key mode). */
+static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, hcs_pair_out* out,
+                     hcs_faces* faces, int64_t* keys, int64_t key_cap, int64_t* key_count) {
+  Plan plan;
+  int rc = stage_pairs(c, npairs, in, opts, plan);
+  if (rc) return rc;
+  const int nitems = (int)plan.items.size();
+  Pipe& P = c->pipe;
+  if ((rc = pipe_reserve(c, P, nitems, npairs, std::max<int64_t>(P.cap, 4096), 0))) return rc;
+  if ((rc = ensure(c, c->d_face_cnt, 64)) || (rc = ensure(c, c->d_poly_cnt, 64)) || (rc = ensure(c, c->d_cand_cnt, 64)))
     return rc;
   const bool keep = opts && opts->keep_polygons && !keys;
   const bool want_faces = faces && faces->capacity > 0 && faces->rec && faces->key && !keys;
@@ -706,32 +823,19 @@ static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_o
   if (keys && (rc = ensure(c, c->d_cand, sizeof(int64_t) * std::max<int64_t>(key_cap, 1)))) return rc;
   int64_t poly_cap = 0;
   if (keep) {
-    /* capacity grows until it holds the step's polygons (rerun below) */
     poly_cap = std::max<int64_t>(c->d_poly_rec.bytes / (sizeof(double) * HCS_PREC), 4096);
     if ((rc = ensure(c, c->d_poly_rec, sizeof(double) * HCS_PREC * poly_cap)) ||
         (rc = ensure(c, c->d_poly_key, sizeof(int64_t) * 3 * poly_cap)))
       return rc;
   }
-  HCHK(c, hipMemcpyAsync(c->d_envs.ptr, plan.envs.data(), sizeof(EnvDesc) * npairs, hipMemcpyHostToDevice,
-                         c->stream));
-  HCHK(c, hipMemcpyAsync(c->d_state.ptr, state.data(), state.size() * 8, hipMemcpyHostToDevice, c->stream));
-  if (nitems)
-    HCHK(c, hipMemcpyAsync(c->d_items.ptr, plan.items.data(), sizeof(Item) * nitems, hipMemcpyHostToDevice,
-                           c->stream));
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    HCHK(c, hipMemsetAsync(c->d_err.ptr, 0, 64, c->stream));
+  std::vector<PairOut> res(npairs);
+  unsigned long long nface = 0, ncand = 0, npoly = 0, nsurv = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
     HCHK(c, hipMemsetAsync(c->d_face_cnt.ptr, 0, 64, c->stream));
     HCHK(c, hipMemsetAsync(c->d_poly_cnt.ptr, 0, 64, c->stream));
     HCHK(c, hipMemsetAsync(c->d_cand_cnt.ptr, 0, 64, c->stream));
-    QueryArgs a{};
-    a.envs = (const EnvDesc*)c->d_envs.ptr;
-    a.state = (const double*)c->d_state.ptr;
-    a.xf = (const double*)c->d_xf.ptr;
-    a.items = (const Item*)c->d_items.ptr;
-    a.nitems = nitems;
-    a.queries_per_lane = 1;
-    a.counter = nullptr;
-    a.partials = (Partial*)c->d_partials.ptr;
+    QueryArgs a = pipe_args(P, (const EnvDesc*)c->d_envs.ptr, (const double*)c->d_state.ptr,
+                            (const Item*)c->d_items.ptr, nitems, false);
     if (want_faces) {
       a.face_rec = (double*)c->d_face_rec.ptr;
       a.face_key = (int64_t*)c->d_face_key.ptr;
@@ -749,33 +853,30 @@ static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_o
       a.cand_cnt = (unsigned long long*)c->d_cand_cnt.ptr;
       a.cand_cap = key_cap;
     }
-    a.error = (int*)c->d_err.ptr;
-    launch_prep(c->stream, a.envs, a.state, (double*)c->d_xf.ptr, npairs);
-    launch_query(c->stream, a, nitems);
+    pipe_enqueue(c, P, a, npairs, nitems, clip_grid_for(c, P.cap), 0, nullptr, nullptr, keys != nullptr, 0);
     HCHK(c, hipGetLastError());
-    launch_reduce(c->stream, a.envs, a.partials, a.error, (PairOut*)c->d_out.ptr, npairs);
-    HCHK(c, hipGetLastError());
-    unsigned long long npoly = 0;
-    if (keep) {
-      HCHK(c, hipMemcpyAsync(&npoly, c->d_poly_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
-      HCHK(c, hipStreamSynchronize(c->stream));
-      if ((int64_t)npoly > poly_cap && attempt == 0) {
-        poly_cap = (int64_t)npoly + npoly / 4 + 1024;
-        if ((rc = ensure(c, c->d_poly_rec, sizeof(double) * HCS_PREC * poly_cap)) ||
-            (rc = ensure(c, c->d_poly_key, sizeof(int64_t) * 3 * poly_cap)))
-          return rc;
-        continue;
-      }
-      c->last_npoly = (int64_t)npoly;
+    HCHK(c, hipMemcpyAsync(&nsurv, P.surv_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+    HCHK(c, hipMemcpyAsync(&npoly, c->d_poly_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+    HCHK(c, hipMemcpyAsync(&nface, c->d_face_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+    HCHK(c, hipMemcpyAsync(&ncand, c->d_cand_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+    if (!keys)
+      HCHK(c, hipMemcpyAsync(res.data(), P.out.ptr, sizeof(PairOut) * npairs, hipMemcpyDeviceToHost, c->stream));
+    HCHK(c, hipStreamSynchronize(c->stream));
+    bool again = false;
+    if (!keys && (int64_t)nsurv > P.cap) {
+      if ((rc = pipe_reserve(c, P, nitems, npairs, (int64_t)nsurv + (int64_t)nsurv / 4 + 1024, 0))) return rc;
+      again = true;
     }
-    break;
+    if (keep && (int64_t)npoly > poly_cap) {
+      poly_cap = (int64_t)npoly + (int64_t)npoly / 4 + 1024;
+      if ((rc = ensure(c, c->d_poly_rec, sizeof(double) * HCS_PREC * poly_cap)) ||
+          (rc = ensure(c, c->d_poly_key, sizeof(int64_t) * 3 * poly_cap)))
+        return rc;
+      again = true;
+    }
+    if (!again) break;
   }
-  std::vector<PairOut> res(npairs);
-  HCHK(c, hipMemcpyAsync(res.data(), c->d_out.ptr, sizeof(PairOut) * npairs, hipMemcpyDeviceToHost, c->stream));
-  unsigned long long nface = 0, ncand = 0;
-  HCHK(c, hipMemcpyAsync(&nface, c->d_face_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
-  HCHK(c, hipMemcpyAsync(&ncand, c->d_cand_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream));
-  HCHK(c, hipStreamSynchronize(c->stream));
+  if (keep) c->last_npoly = (int64_t)npoly;
   if (want_faces) {
     const int64_t nw = std::min<int64_t>((int64_t)nface, faces->capacity);
     if (nw > 0) {
@@ -789,6 +890,9 @@ static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_o
   } else if (faces) {
     faces->count = 0;
   }
+  int err = 0;
+  HCHK(c, hipMemcpyAsync(&err, P.err.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HCHK(c, hipStreamSynchronize(c->stream));
   if (keys) {
     const int64_t nw = std::min<int64_t>((int64_t)ncand, key_cap);
     if (nw > 0) {
@@ -797,21 +901,19 @@ static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_o
       std::sort(keys, keys + nw);
     }
     if (key_count) *key_count = (int64_t)ncand;
+    if (err) return set_err(c, HCS_ERR_LIMIT, "traversal stack overflow");
+    if ((int64_t)ncand > key_cap)
+      return set_err(c, HCS_ERR_CAPACITY, "key buffer too small: " + std::to_string(ncand) + " keys");
+    return HCS_OK;
   }
   if (out) std::memcpy(out, res.data(), sizeof(PairOut) * npairs);
-  int status = HCS_OK;
-  for (auto& r : res)
-    if (r.status) status = (int)r.status;
   if (keep) {
     c->last_valid = 1;
     c->last_pairs.assign(in, in + npairs);
-    c->last_geoms = geoms;
   }
-  if (status == HCS_ERR_LIMIT) return set_err(c, status, "traversal stack overflow");
+  if (err) return set_err(c, HCS_ERR_LIMIT, "traversal stack overflow");
   if (want_faces && (int64_t)nface > faces->capacity)
     return set_err(c, HCS_ERR_CAPACITY, "face buffer too small: " + std::to_string(nface) + " faces");
-  if (keys && (int64_t)ncand > key_cap)
-    return set_err(c, HCS_ERR_CAPACITY, "key buffer too small: " + std::to_string(ncand) + " keys");
   return HCS_OK;
 }
 

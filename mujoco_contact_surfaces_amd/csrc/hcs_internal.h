@@ -33,6 +33,14 @@ struct DBuf {
   size_t bytes = 0;
 };
 
+/* The per-step pipeline (hcs_engine.hip): k_prep, k_cand, scan of
+ * per-item survivor counts, k_clip, k_env_reduce. */
+struct Pipe {
+  DBuf counter, surv, surv_cnt, ncand, nsurv, off, scan_tmp, contrib, out, err, xf;
+  int64_t cap = 0;
+  size_t scan_bytes = 0;
+};
+
 struct Ctx;
 int set_err(Ctx* c, int code, const std::string& msg);
 int hip_check(Ctx* c, hipError_t e, const char* what);
@@ -52,7 +60,8 @@ struct Ctx {
   std::string err;
   std::vector<Mesh> meshes;
   /* scratch for hcs_contact_step / hcs_candidates / hcs_taxels */
-  DBuf d_envs, d_state, d_xf, d_items, d_partials, d_out, d_err;
+  DBuf d_envs, d_state, d_items;
+  Pipe pipe;
   DBuf d_face_rec, d_face_key, d_face_cnt;
   DBuf d_poly_rec, d_poly_key, d_poly_cnt;
   DBuf d_cand, d_cand_cnt;
@@ -70,16 +79,21 @@ struct Ctx {
 int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int queries_per_lane,
                int tet_mode_max_tris, Plan& plan);
 
-/* Launchers (hcs_engine.hip). */
+int pipe_reserve(Ctx* c, Pipe& p, int nitems, int npairs, int64_t cap, int ndir);
+QueryArgs pipe_args(const Pipe& p, const EnvDesc* envs, const double* state, const Item* items, int nitems,
+                    bool persistent);
+/* ndir = 0: forward (double); ndir > 0: tangent directions dir0 ..
+ * dir0+ndir-1 (contributions and outputs indexed by direction - dir0). */
+void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid, int clip_grid, int ndir,
+                  const double* dstar_dE, const double* invE, bool cand_only, int dir0);
+void pipe_free(Pipe& p);
+void launch_cand(hipStream_t s, const QueryArgs& a, int grid);
+void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, hipEvent_t e_clip,
+                        hipEvent_t e_reduce);
+int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan);
+int clip_grid_for(const Ctx* c, int64_t cap);
 void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double* xf, int n);
-void launch_query(hipStream_t s, const QueryArgs& a, int grid);
-void launch_reduce(hipStream_t s, const EnvDesc* envs, const Partial* partials, const int* err, PairOut* out,
-                   int n);
 void launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, int n);
-void launch_query_tangent_dir(hipStream_t s, const QueryArgs& a, int dir, const double* dstar_dE,
-                              const double* invE, Partial* tpart);
-void launch_query_tangent(hipStream_t s, const QueryArgs& a, int ndir, const double* dstar_dE, const double* invE,
-                          Partial* tpart);
 
 struct TaxelGridDev {
   double X[12];
@@ -89,8 +103,6 @@ struct TaxelGridDev {
 int taxel_grid_dev(const hcs_taxel_grid& G, TaxelGridDev& g);
 int run_taxels(Ctx* c, const TaxelGridDev& g, const unsigned char* d_mask, const double* poly, const double* ptan,
                const int64_t* key, int64_t np, const double* Xd, double* d_out);
-/* Uploads the plan/state of npairs pairs into the context scratch buffers. */
-int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan);
 
 }  // namespace hcs
 

@@ -58,11 +58,6 @@ struct Item {
   int32_t env, first, count, pad;
 };
 
-struct Partial {
-  double F[3], tA[3], tB[3];
-  int64_t nc, np, nf;
-};
-
 struct PairOut {
   double F_A[3], tau_A[3], F_B[3], tau_B[3];
   int64_t n_candidates, n_polygons, n_faces, status;
@@ -76,7 +71,6 @@ struct QueryArgs {
   int32_t nitems;
   int32_t queries_per_lane;
   unsigned int* counter; /* persistent work queue; nullptr: item = blockIdx.x */
-  Partial* partials;
   double* face_rec;
   int64_t* face_key;
   unsigned long long* face_cnt;
@@ -93,6 +87,21 @@ struct QueryArgs {
   /* optional per-item profile: cycles, production steps, candidates,
    * survivors of the cheap tests (4 x uint64 per item) */
   unsigned long long* item_stats;
+  /* survivors of the cheap tests: (elem A, elem B, item, seq in item) */
+  int4* surv;
+  unsigned long long* surv_cnt;
+  int64_t surv_cap;
+  int* item_ncand;
+  unsigned* item_nsurv;
+};
+
+/* Inputs of the clip kernel beyond QueryArgs. */
+struct ClipArgs {
+  const unsigned* item_off; /* exclusive scan of item_nsurv, nitems + 1 */
+  const double* dstar_dE;   /* tangent seeds (tangent pass only) */
+  const double* invE;
+  int64_t cap; /* contribution slots per direction; a survivor whose slot
+                  is beyond it (survivor list overflow) writes nothing */
 };
 
 }  // namespace hcs

@@ -1,20 +1,25 @@
 /*
- * hcs_query.h -- the fused per-step contact kernel body (device code),
- * templated on the scalar type: double for the forward step, Dual for the
- * tangent-linear backward (one tangent direction per grid.y slice).
+ * hcs_query.h -- device bodies of the per-step contact pipeline, templated on
+ * the scalar type where they do arithmetic on poses (double for the forward
+ * step, Dual for the tangent-linear pass, one direction per grid.y slice).
  *
- * One wavefront owns one work item (a chunk of queries of one pair):
- *   1. each lane builds the AABB of its query element in the soft frame A
- *      (a rigid triangle moved by X_AB, a tet of soft body B, or a tet of A
- *      when the rigid side is tiny and is brute-forced from LDS);
- *   2. lanes walk A's LBVH (stack in LDS); leaf hits become candidate pairs
- *      that are compacted into an LDS queue with a wave ballot + mbcnt
- *      prefix count;
- *   3. whenever 64 candidates are queued, every lane takes one: clip
- *      (registers only), faces, force law, wrench accumulation;
- *   4. the item's wrench is reduced across the wave in a fixed butterfly
- *      order and stored at partials[item], so results do not depend on which
- *      wave ran the item or in what order items were taken.
+ *   1. run_cand: one wavefront owns one work item (a chunk of queries of one
+ *      pair).  Each lane builds the AABB of its query element in the soft
+ *      frame A (a rigid triangle moved by X_AB, or a tet of soft body B); in
+ *      tet-query mode the wave walks A's Morton-ordered tets in tiles culled
+ *      against the union box of the item's queries instead.  Lanes walk A's
+ *      LBVH on private stacks; lanes that run dry take the bottom entry of a
+ *      donor's stack (intra-wave work stealing).  Leaf hits go through the
+ *      exact cheap rejections of survives(); survivors are compacted with a
+ *      ballot + mbcnt prefix count and appended to the survivor list with a
+ *      per-item sequence number.
+ *   2. an exclusive scan over the per-item survivor counts gives each item
+ *      its slot range, so contribution i of item k lands at off[k] + seq.
+ *   3. clip_survivor: one survivor per thread -- clip, faces, force law,
+ *      wrench contribution.
+ *   4. per pair, contributions are summed lane-strided in slot order then by
+ *      a fixed butterfly, so results do not depend on which wave ran which
+ *      item or in what order survivors were appended.
  * Reference semantics per candidate: see hcs_device.h.
  */
 #pragma once
@@ -427,7 +432,7 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
  * soft-soft: the equal-pressure plane, both gradient filters and the
  * marching-tets sign test of process_candidate (same arithmetic). */
 template <class T>
-__device__ inline bool survives(const EnvDesc& E, const ItemCtx<T>& ic, const double* X, int a, int b) {
+__device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b) {
   const double* r = E.recA + (size_t)HCS_TREC * a;
   if (E.kind != KIND_SOFT) {
     const int32_t* f = E.FB + 3 * (size_t)b;
@@ -458,7 +463,7 @@ __device__ inline bool survives(const EnvDesc& E, const ItemCtx<T>& ic, const do
     return !(fabs(dist) > rad + 1e-12);
   }
   const double* rb = E.recB + (size_t)HCS_TREC * b;
-  const double sc = val(ic.escale);
+  const double sc = 1.0;
   double gA[3] = {sc * r[16], sc * r[17], sc * r[18]};
   double cA = sc * r[19];
   double gbB[3] = {rb[16], rb[17], rb[18]}, ghB[3] = {rb[20], rb[21], rb[22]};
@@ -483,56 +488,53 @@ __device__ inline bool survives(const EnvDesc& E, const ItemCtx<T>& ic, const do
   return above != 0 && above != 4;
 }
 
-/* Shared-memory layout of one wave: candidate queue (traversal output),
- * survivor queue (input of the full clip), traversal stacks, a tile of
- * rigid-triangle boxes for the tet-query mode, and the per-item context
- * (poses, twists, material constants) read by every lane. */
-template <class T>
-struct WaveLds {
+/* ===================================================================
+ * k_cand: candidate generation, exact cheap tests, survivor compaction
+ * =================================================================== */
+
+/* Per-wave shared memory: candidate queue, survivor queue, rigid-triangle
+ * tile (tet queries) or query boxes per slot (triangle / soft queries), the
+ * work-donation mailbox and the pair transform. */
+struct CandLds {
   int2 q1[192];
   int2 q2[128];
-  int stk[HCS_STACK][64];
-  double tile[64 * 6];
-  ItemCtx<T> ic;
+  double box[64 * 6];
+  int2 xfer[64];
   double X[12];
 };
 
-/* Body of the query kernel for one item: one loop that alternates one
- * production step (a traversal step, or one rigid triangle tested against
- * every lane's tet) with the consumers (cheap tests on 64 queued candidates,
- * full clip of 64 survivors), so each stage is instantiated once. */
-template <class T>
-__device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int dir, const double* dstar_dE,
-                                const double* invE, Partial* out, const OutPtrs* o) {
+/* One item: produce candidates (traversal with intra-wave work stealing, or
+ * rigid-triangle tiles against 64 Morton-ordered tets), run the exact cheap
+ * tests on every 64 queued candidates, and append survivors to HBM in
+ * groups of up to 64 with a per-item sequence number, so the clip kernel can
+ * place each contribution at item_off[item] + seq whatever order the groups
+ * land in. */
+__device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
   const int lane = threadIdx.x & 63;
   const unsigned long long t_begin = A.item_stats ? clock64() : 0ull;
-  unsigned long long steps = 0, survivors = 0;
+  unsigned long long steps = 0;
   const Item item = A.items[it];
   const int env = item.env;
   const EnvDesc E = A.envs[env];
-  const double* st = A.state + (size_t)HCS_STATE * env;
   __syncthreads();
-  if (lane == 0) {
-    make_item_ctx<T>(E, st, dir, dstar_dE ? dstar_dE[env] : 0.0, invE ? invE[env] : 0.0, L.ic);
-#pragma unroll
-    for (int i = 0; i < 12; ++i) L.X[i] = val(L.ic.X_AB[i]);
-  }
+  if (lane < 12) L.X[lane] = A.xf[12 * (size_t)env + lane];
   __syncthreads();
-  const ItemCtx<T>& ic = L.ic;
   const double* X = L.X;
-  Acc<T> acc;
-  acc.zero();
-  int64_t nc = 0, np = 0, nf = 0;
+  int nc = 0;
+  unsigned nsurv = 0;
   int q1n = 0, q2n = 0;
   const bool tet_mode = E.kind == KIND_TET;
-  /* producer state */
   bool producing = true;
-  int sub = -1, sp = 0, visits = 0, b = 0;
-  bool valid = false;
+  /* traversal state (triangle / soft queries) */
+  int stk[HCS_STACK];
+  int bp = 0, sp = 0, visits = 0, b = 0;
+  const int visit_cap = E.ntA * 64;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  /* tile state (tet queries) */
   int t0 = 0, ntile = 0;
   unsigned long long tmask = 0;
   bool tile_loaded = false;
+  bool valid = false;
   double bx[6] = {0, 0, 0, 0, 0, 0}, ulo[3] = {0, 0, 0}, uhi[3] = {0, 0, 0};
   if (tet_mode) {
     /* the lane's tet: 64 consecutive leaves in Morton order, so the item's
@@ -545,6 +547,33 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
       bx[3 + c] = valid ? E.boxA[6 * (size_t)b + 3 + c] : -__builtin_inf();
       ulo[c] = wave_min(bx[c]);
       uhi[c] = wave_max(bx[3 + c]);
+    }
+  } else {
+    valid = lane < item.count;
+    b = item.first + lane;
+    if (valid) {
+      const int nvq = E.kind == KIND_SOFT ? 4 : 3;
+      const int32_t* f = E.FB + (size_t)nvq * b;
+      for (int i = 0; i < nvq; ++i) {
+        double y[3];
+        xform(X, E.VB + 3 * (size_t)f[i], y);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
+          hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = lo[c], L.box[6 * lane + 3 + c] = hi[c];
+    }
+    if (E.ntA == 1) {
+      const bool hit = valid && box_overlap(E.boxA, lo, hi);
+      const unsigned long long m = __ballot(hit);
+      if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(0, b);
+      q1n += __popcll(m);
+    } else if (valid) {
+      stk[0] = 0;
+      sp = 1;
     }
   }
   for (;;) {
@@ -577,7 +606,7 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
                 }
               }
 #pragma unroll
-              for (int c = 0; c < 3; ++c) L.tile[6 * lane + c] = l3[c], L.tile[6 * lane + 3 + c] = h3[c];
+              for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = l3[c], L.box[6 * lane + 3 + c] = h3[c];
               const double ub[6] = {ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2]};
               touch = box_overlap(ub, l3, h3);
             }
@@ -587,82 +616,82 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
         } else {
           const int jj = __builtin_ctzll(tmask);
           tmask &= tmask - 1;
-          const double* tb = &L.tile[6 * jj];
+          const double* tb = &L.box[6 * jj];
           const bool hit = valid && box_overlap(bx, tb, tb + 3);
           const unsigned long long m = __ballot(hit);
           if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, t0 + jj);
           q1n += __popcll(m);
         }
-      } else if (!__any(sp > 0)) {
-        /* next query of this lane, or done */
-        ++sub;
-        if (sub >= A.queries_per_lane) {
+      } else {
+        int cnt = sp - bp;
+        if (!__any(cnt > 0)) {
           producing = false;
         } else {
-          const int qi = sub * 64 + lane;
-          valid = qi < item.count;
-          b = item.first + qi;
-          if (valid) {
-            const int nvq = E.kind == KIND_SOFT ? 4 : 3;
-            const int32_t* f = E.FB + (size_t)nvq * b;
-            for (int i = 0; i < nvq; ++i) {
-              double y[3];
-              xform(X, E.VB + 3 * (size_t)f[i], y);
-#pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
-                hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+          /* work stealing: the k-th idle lane takes the bottom (shallowest,
+           * largest) pending subtree of the k-th lane holding two or more */
+          const unsigned long long idle = __ballot(cnt == 0), donors = __ballot(cnt >= 2);
+          const int npair = min(__popcll(idle), __popcll(donors));
+          if (npair > 0) {
+            if (cnt >= 2) {
+              const int r = (int)lane_prefix(donors);
+              if (r < npair) {
+                L.xfer[r] = make_int2(b - item.first, stk[bp % HCS_STACK]);
+                ++bp;
+                --cnt;
               }
             }
-          }
-          visits = 0;
-          sp = 0;
-          if (E.ntA == 1) {
-            const bool hit = valid && box_overlap(E.boxA, lo, hi);
-            const unsigned long long m = __ballot(hit);
-            if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(0, b);
-            q1n += __popcll(m);
-          } else if (valid) {
-            L.stk[0][lane] = 0;
-            sp = 1;
-          }
-        }
-      } else {
-        /* one traversal step; a tree of ntA leaves has ntA-1 internal
-         * nodes, each popped at most once per query (the cap only guards
-         * against a malformed tree) */
-        int c0 = -1, c1 = -1;
-        if (sp > 0 && ++visits > E.ntA) {
-          atomicOr(A.error, 2);
-          sp = 0;
-        }
-        if (sp > 0) {
-          const int nd = L.stk[--sp][lane];
-          const Node* N = E.nodesA + nd;
-          double bb[12];
+            __syncthreads();
+            if (cnt == 0) {
+              const int r = (int)lane_prefix(idle);
+              if (r < npair) {
+                const int2 w = L.xfer[r];
+                b = item.first + w.x;
 #pragma unroll
-          for (int i = 0; i < 12; ++i) bb[i] = N->b[i];
-          const int ch0 = N->c[0], ch1 = N->c[1];
-          const bool o0 = box_overlap(bb, lo, hi), o1 = box_overlap(bb + 6, lo, hi);
-          if (o1) {
-            if (ch1 < 0) c1 = ~ch1;
-            else if (sp < HCS_STACK) L.stk[sp++][lane] = ch1;
-            else atomicOr(A.error, 1);
+                for (int c = 0; c < 3; ++c) lo[c] = L.box[6 * w.x + c], hi[c] = L.box[6 * w.x + 3 + c];
+                bp = sp = 0;
+                stk[0] = w.y;
+                sp = 1;
+                cnt = 1;
+              }
+            }
+            __syncthreads();
           }
-          if (o0) {
-            if (ch0 < 0) c0 = ~ch0;
-            else if (sp < HCS_STACK) L.stk[sp++][lane] = ch0;
-            else atomicOr(A.error, 1);
+          /* one traversal step (a tree of ntA leaves has ntA-1 internal
+           * nodes; the visit cap only guards against a malformed tree) */
+          int c0 = -1, c1 = -1;
+          if (cnt > 0 && ++visits > visit_cap) {
+            atomicOr(A.error, 2);
+            bp = sp = 0;
+            cnt = 0;
           }
+          if (cnt > 0) {
+            const int nd = stk[(--sp) % HCS_STACK];
+            const Node* N = E.nodesA + nd;
+            double bb[12];
+#pragma unroll
+            for (int i = 0; i < 12; ++i) bb[i] = N->b[i];
+            const int ch0 = N->c[0], ch1 = N->c[1];
+            const bool o0 = box_overlap(bb, lo, hi), o1 = box_overlap(bb + 6, lo, hi);
+            if (o1) {
+              if (ch1 < 0) c1 = ~ch1;
+              else if (sp - bp < HCS_STACK) stk[(sp++) % HCS_STACK] = ch1;
+              else atomicOr(A.error, 1);
+            }
+            if (o0) {
+              if (ch0 < 0) c0 = ~ch0;
+              else if (sp - bp < HCS_STACK) stk[(sp++) % HCS_STACK] = ch0;
+              else atomicOr(A.error, 1);
+            }
+          }
+          const unsigned long long m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
+          if (c0 >= 0) L.q1[q1n + lane_prefix(m0)] = make_int2(c0, b);
+          if (c1 >= 0) L.q1[q1n + __popcll(m0) + lane_prefix(m1)] = make_int2(c1, b);
+          q1n += __popcll(m0) + __popcll(m1);
         }
-        const unsigned long long m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
-        if (c0 >= 0) L.q1[q1n + lane_prefix(m0)] = make_int2(c0, b);
-        if (c1 >= 0) L.q1[q1n + __popcll(m0) + lane_prefix(m1)] = make_int2(c1, b);
-        q1n += __popcll(m0) + __popcll(m1);
       }
     }
-    /* ---- consume until both queues hold fewer than 64 (or are empty at
-     * the end); q1 < 192 and q2 < 128 hold throughout */
+    /* ---- consume: cheap tests on 64 candidates, flush 64 survivors;
+     * q1 < 192 and q2 < 128 hold throughout */
     for (;;) {
       const bool do2 = q2n >= 64 || (!producing && q1n == 0 && q2n > 0);
       const bool do1 = q1n >= 64 || (!producing && q1n > 0);
@@ -671,10 +700,15 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
       if (do2) {
         const int take = q2n >= 64 ? 64 : q2n;
         const int base = q2n - take;
+        unsigned long long slot0 = 0;
+        if (lane == 0) slot0 = atomicAdd(A.surv_cnt, (unsigned long long)take);
+        slot0 = __shfl(slot0, 0, 64);
         if (lane < take) {
           const int2 c = L.q2[base + lane];
-          process_candidate<T>(E, ic, env, c.x, c.y, acc, np, nf, o);
+          const unsigned long long slot = slot0 + lane;
+          if ((int64_t)slot < A.surv_cap) A.surv[slot] = make_int4(c.x, c.y, it, (int)(nsurv + lane));
         }
+        nsurv += take;
         q2n = base;
       } else {
         const int take = q1n >= 64 ? 64 : q1n;
@@ -688,7 +722,7 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
             unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
             if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
           } else {
-            keep = survives<T>(E, ic, X, c.x, c.y);
+            keep = survives<double>(E, X, c.x, c.y);
           }
         }
         q1n = base;
@@ -696,36 +730,55 @@ __device__ inline void run_item(WaveLds<T>& L, const QueryArgs& A, int it, int d
         __syncthreads();
         if (keep) L.q2[q2n + lane_prefix(m)] = c;
         q2n += __popcll(m);
-        survivors += __popcll(m);
       }
       __syncthreads();
     }
     if (!producing) break;
   }
-  /* fixed-order wave reduction */
-  Partial r;
-  T F[3], tA[3], tB[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    F[c] = wave_sum<T>(acc.F[c]);
-    tA[c] = wave_sum<T>(acc.tA[c]);
-    tB[c] = wave_sum<T>(acc.tB[c]);
-  }
-  const int64_t snc = wave_sum_i(nc), snp = wave_sum_i(np), snf = wave_sum_i(nf);
+  const int snc = (int)wave_sum_i(nc);
   if (lane == 0) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      r.F[c] = dir < 0 ? val(F[c]) : tangent(F[c]);
-      r.tA[c] = dir < 0 ? val(tA[c]) : tangent(tA[c]);
-      r.tB[c] = dir < 0 ? val(tB[c]) : tangent(tB[c]);
-    }
-    r.nc = snc, r.np = snp, r.nf = snf;
-    *out = r;
+    A.item_ncand[it] = snc;
+    A.item_nsurv[it] = nsurv;
     if (A.item_stats) {
       unsigned long long* q = A.item_stats + 4 * (size_t)it;
-      q[0] = clock64() - t_begin, q[1] = steps, q[2] = (unsigned long long)snc, q[3] = survivors;
+      q[0] = clock64() - t_begin, q[1] = steps, q[2] = (unsigned long long)snc, q[3] = nsurv;
     }
   }
+}
+
+/* ===================================================================
+ * k_clip: one survivor per thread -> contribution at item_off + seq
+ * =================================================================== */
+
+struct Contrib {
+  double F[3], tA[3], tB[3];
+  int np, nf;
+};
+
+template <class T>
+__device__ inline void clip_survivor(const QueryArgs& A, const ClipArgs& C, int64_t i, int dir, Contrib* out,
+                                     const OutPtrs* o) {
+  const int4 s = A.surv[i];
+  const int64_t slot = (int64_t)C.item_off[s.z] + (unsigned)s.w;
+  if (slot >= C.cap) return;
+  const int env = A.items[s.z].env;
+  const EnvDesc E = A.envs[env];
+  ItemCtx<T> ic;
+  make_item_ctx<T>(E, A.state + (size_t)HCS_STATE * env, dir, C.dstar_dE ? C.dstar_dE[env] : 0.0,
+                   C.invE ? C.invE[env] : 0.0, ic);
+  Acc<T> acc;
+  acc.zero();
+  int64_t np = 0, nf = 0;
+  process_candidate<T>(E, ic, env, s.x, s.y, acc, np, nf, o);
+  Contrib r;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    r.F[c] = dir < 0 ? val(acc.F[c]) : tangent(acc.F[c]);
+    r.tA[c] = dir < 0 ? val(acc.tA[c]) : tangent(acc.tA[c]);
+    r.tB[c] = dir < 0 ? val(acc.tB[c]) : tangent(acc.tB[c]);
+  }
+  r.np = (int)np, r.nf = (int)nf;
+  out[slot] = r;
 }
 
 }  // namespace hcs

@@ -271,7 +271,7 @@ class Batch:
     def kernel_times(self, reps=5):
         ms = np.zeros(3)
         self.e._chk(self.e.lib.hcs_batch_kernel_times(self.b, int(reps), A.dptr(ms)))
-        return dict(prep=ms[0], query=ms[1], reduce=ms[2])
+        return dict(cand=ms[0], clip=ms[1], rest=ms[2])
 
     def item_stats(self):
         """Per-item profile of one step: columns pair, kind, queries, cycles,
