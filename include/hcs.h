@@ -116,6 +116,8 @@ typedef struct hcs_taxel_grid {
 
 /* ---- lifetime (plugin constructor/load/destructor, plugin.cpp:191-245) */
 int hcs_create(hcs_ctx** ctx, int device_ordinal);
+/* Also destroys the context's remaining batches; their handles become
+ * invalid. */
 int hcs_destroy(hcs_ctx* ctx);
 const char* hcs_last_error(const hcs_ctx* ctx); /* NULL ctx: last create error */
 int hcs_synchronize(hcs_ctx* ctx);

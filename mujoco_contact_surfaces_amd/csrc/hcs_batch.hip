@@ -123,6 +123,7 @@ int hcs_batch_create(hcs_ctx* h, int32_t npairs, const int32_t* geoms, const hcs
   if (rc) return rc;
   hcs_batch* b = new hcs_batch();
   b->c = c;
+  c->batches.insert(b);
   b->npairs = npairs;
   b->geoms.assign(geoms, geoms + 2 * (size_t)npairs);
   if (opts) {
@@ -156,6 +157,7 @@ int hcs_batch_destroy(hcs_batch* b) {
   for (DBuf* d : {&b->envs, &b->state, &b->items})
     if (d->ptr) (void)hipFree(d->ptr);
   pipe_free(b->pipe);
+  b->c->batches.erase(b);
   delete b;
   return HCS_OK;
 }

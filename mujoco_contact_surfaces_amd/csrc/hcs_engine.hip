@@ -659,6 +659,7 @@ int hcs_destroy(hcs_ctx* h) {
   Ctx* c = &h->c;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  while (!c->batches.empty()) hcs_batch_destroy(*c->batches.begin());
   for (auto& m : c->meshes) free_mesh(m);
   pipe_free(c->pipe);
   for (DBuf* b : {&c->d_envs, &c->d_state, &c->d_items, &c->d_face_rec, &c->d_face_key, &c->d_face_cnt, &c->d_poly_rec, &c->d_poly_key, &c->d_poly_cnt,

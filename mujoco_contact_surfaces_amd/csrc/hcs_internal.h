@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <set>
 #include <string>
 #include <vector>
 
@@ -72,6 +73,8 @@ struct Ctx {
   std::vector<hcs_pair_in> last_pairs;
   std::vector<int32_t> last_geoms; /* per pair: geom_a, geom_b */
   int tet_mode_max_tris = 64;
+  /* live batches of this context: hcs_destroy releases them first */
+  std::set<hcs_batch*> batches;
 };
 
 /* Builds env descriptors and work items; tri-query chunks of 64 *

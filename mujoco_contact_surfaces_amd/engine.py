@@ -13,6 +13,7 @@ geometry, sensors sample the kept contact polygons); Engine is the direct
 C-ABI wrapper.  Everything runs through libhcs.so; there is no CPU path.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -88,6 +89,7 @@ class Engine:
             raise HcsError(rc, self.lib.hcs_last_error(None).decode())
         self.h = h
         self._kind = {}
+        self._batches = weakref.WeakSet()
 
     def _chk(self, rc):
         if rc != 0:
@@ -95,6 +97,9 @@ class Engine:
 
     def close(self):
         if self.h:
+            # a batch must not outlive its context: release them first
+            for b in list(self._batches):
+                b.close()
             self.lib.hcs_destroy(self.h)
             self.h = None
 
@@ -234,6 +239,8 @@ class Batch:
 
     def __init__(self, engine, geoms, opts):
         self.e = engine
+        self.b = None
+        engine._batches.add(self)
         g = A.as_i32(geoms, (-1, 2))
         self.n = g.shape[0]
         b = C.c_void_p()
