@@ -77,43 +77,38 @@ def algorithmic_bytes(engine, scs, handles, counts, survivors, nitems):
                                                     polygons=int(counts[:, 1].sum()))
 
 
-def cpu_baseline(scs, budget_s=12.0):
-    """Oracle per-step work (load-time BVH excluded), serial, on a bounded
-    prefix of the environments; then the same prefix on all host threads."""
+def cpu_baseline(scs, budget_s=12.0, chunk=128):
+    """Oracle per-step work (load-time BVH excluded) on the environments in
+    order, chunk by chunk, until about budget_s of serial step time: each
+    chunk is prepared (untimed), stepped serially (timed), stepped again on
+    all host threads (timed), then freed, so host memory stays bounded."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from concurrent.futures import ThreadPoolExecutor
 
-    preps, args = [], []
-    t_prep = time.time()
-    for sc in scs:
-        s, r = sc["soft"], sc["rigid"]
-        preps.append(O.PreparedSoft(s["V"], s["T"], s["p"]))
-        pose = np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]])
-        params = [s["props"]["hydroelastic_modulus"], s["props"]["dissipation"], s["props"]["dynamic_friction"],
-                  r["props"]["dynamic_friction"], 0.001, 1, 0, 1]
-        args.append((r["V"], r["F"], pose, params))
-        if time.time() - t_prep > budget_s:
-            break
-    n = 0
-    clips = 0
-    t0 = time.perf_counter()
-    while n < len(preps) and time.perf_counter() - t0 < budget_s:
-        _, c = preps[n].step(*args[n])
-        clips += int(c[0])
-        n += 1
-    serial = time.perf_counter() - t0
     threads = min(16, os.cpu_count() or 1)
-
-    def run(i):
-        return int(preps[i].step(*args[i])[1][0])
-
-    t1 = time.perf_counter()
+    clips = par_clips = n = 0
+    serial = par = 0.0
     with ThreadPoolExecutor(threads) as ex:
-        par_clips = sum(ex.map(run, range(n)))
-    par = time.perf_counter() - t1
-    for p in preps:
-        p.close()
+        while n < len(scs) and serial < budget_s:
+            preps, args = [], []
+            for sc in scs[n:n + chunk]:
+                s, r = sc["soft"], sc["rigid"]
+                preps.append(O.PreparedSoft(s["V"], s["T"], s["p"]))
+                pose = np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]])
+                params = [s["props"]["hydroelastic_modulus"], s["props"]["dissipation"],
+                          s["props"]["dynamic_friction"], r["props"]["dynamic_friction"], 0.001, 1, 0, 1]
+                args.append((r["V"], r["F"], pose, params))
+            t0 = time.perf_counter()
+            for p, ar in zip(preps, args):
+                clips += int(p.step(*ar)[1][0])
+            serial += time.perf_counter() - t0
+            t1 = time.perf_counter()
+            par_clips += sum(ex.map(lambda k: int(preps[k].step(*args[k])[1][0]), range(len(preps))))
+            par += time.perf_counter() - t1
+            for p in preps:
+                p.close()
+            n += len(preps)
     return dict(value=clips / serial, unit="clips/s", cores=1, kind="port",
                 sample="first %d of %d envs, %d clips, %.1f s serial, load-time BVH excluded" % (n, len(scs), clips,
                                                                                                     serial),
@@ -130,7 +125,7 @@ def main():
     ap.add_argument("--plain-envs", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--config", default="", help="fixed launch shape: blocks_per_cu,queries_per_lane,tet_max")
+    ap.add_argument("--config", default="", help="fixed launch shape: blocks_per_cu,queries_per_lane,tet_max[,split_target]")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: the engine is single-device (multi-GPU sharding is design-only); "
@@ -147,14 +142,14 @@ def main():
     total_tris = int(sum(sc["rigid"]["F"].shape[0] for sc in scs))
     batch = engine.batch(handles)
     batch.set_state(state)
+    if a.config:
+        batch.set_config(*[int(v) for v in a.config.split(",")])
+        log("fixed config: %s" % batch.config())
     batch.step()
     W0, cnt = batch.get()
     clips = int(cnt[:, 0].sum())
     log("step counts: clips %d polygons %d faces %d" % tuple(cnt.sum(axis=0)))
-    if a.config:
-        batch.set_config(*[int(v) for v in a.config.split(",")])
-        log("fixed config: %s" % batch.config())
-    elif not a.no_tune:
+    if not a.config and not a.no_tune:
         ms = batch.tune(reps=2)
         log("autotuned: %s (%.3f ms)" % (batch.config(), ms))
 
@@ -175,15 +170,17 @@ def main():
 
     # plain per-pair GPU path: one synchronous hcs_contact_step per pair
     npl = min(a.plain_envs, a.envs)
-    pairs = [(ha, hb, sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]) for sc, (ha, hb) in zip(scs[:npl], handles)]
-    for p in pairs[:8]:
-        engine.contact_step([p])
-    t1 = time.perf_counter()
-    pclips = 0
-    for p in pairs:
-        _, c = engine.contact_step([p])
-        pclips += int(c[0, 0])
-    plain = pclips / (time.perf_counter() - t1)
+    plain = None
+    if npl > 0:
+        pairs = [(ha, hb, sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]) for sc, (ha, hb) in zip(scs[:npl], handles)]
+        for p in pairs[:8]:
+            engine.contact_step([p])
+        t1 = time.perf_counter()
+        pclips = 0
+        for p in pairs:
+            _, c = engine.contact_step([p])
+            pclips += int(c[0, 0])
+        plain = pclips / (time.perf_counter() - t1)
 
     st = batch.item_stats()
     survivors = int(st[:, 6].sum())
@@ -204,7 +201,7 @@ def main():
                            envs=a.envs, total_tets=total_tets, total_tris=total_tris, clips_per_step=clips,
                            polygons_per_step=int(cnt[:, 1].sum()), faces_per_step=int(cnt[:, 2].sum()),
                            autotune=batch.config(), parallelism="single GPU"),
-               us_per_step=dt * 1e6, plain_clips_per_s=plain, batched_over_plain=value / plain,
+               us_per_step=dt * 1e6, plain_clips_per_s=plain, batched_over_plain=value / plain if plain else None,
                roofline=roof, cpu_baseline=cpu)
     if cpu:
         out["gpu_over_cpu_serial"] = value / cpu["value"]

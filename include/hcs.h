@@ -171,10 +171,14 @@ int hcs_batch_set_state(hcs_batch* batch, const double* state);
 /* One contact step for every pair (graph replay; asynchronous on hcs_stream). */
 int hcs_batch_step(hcs_batch* batch);
 int hcs_batch_get(hcs_batch* batch, hcs_pair_out* out);
-/* Autotuner: times the launch shapes on the current state, keeps the best. */
+/* Autotuner: times the launch shapes on the current state, keeps the best.
+ * Re-planning (here and in hcs_batch_set_config) measures one step on the
+ * current state and splits heavy work items over subtrees; results never
+ * depend on the plan beyond the order of floating-point sums. */
 int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
 /* Launch configuration: cfg = [persistent blocks per CU, queries per lane,
- * largest rigid mesh handled in tet-query mode]. */
+ * largest rigid mesh handled in tet-query mode, candidate count above which
+ * a work item is split over LBVH subtrees (0: never)]; 4 int32. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
 int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
 /* Eager per-kernel timing with HIP events on hcs_stream, averaged over

@@ -29,6 +29,10 @@ struct hcs_batch {
   hcs_step_opts opts{};
   Plan plan;
   int qpl = 1, blocks_per_cu = 4, tet_max = 64;
+  /* split triangle / soft items whose last measured candidate count exceeds
+   * this into sub-items over LBVH subtrees (0: never) */
+  int split_target = 2048;
+  bool has_state = false;
   DBuf envs, state, items;
   Pipe pipe;
   hipGraph_t graph = nullptr;
@@ -59,13 +63,11 @@ static void batch_enqueue(hcs_batch* b, const QueryArgs& a) {
                false, 0);
 }
 
-/* (Re)builds the plan for the current shape and uploads it. */
-static int batch_plan(hcs_batch* b) {
+/* Uploads envs and items of b->plan and computes the static item boxes. */
+static int batch_upload(hcs_batch* b) {
   Ctx* c = b->c;
-  batch_free_graph(b);
-  int rc = build_plan(c, b->npairs, b->geoms.data(), &b->opts, b->qpl, b->tet_max, b->plan);
-  if (rc) return rc;
   const size_t nitems = std::max<size_t>(1, b->plan.items.size());
+  int rc;
   if ((rc = ensure(c, b->envs, sizeof(EnvDesc) * b->npairs)) ||
       (rc = ensure(c, b->state, sizeof(double) * HCS_STATE * b->npairs)) ||
       (rc = ensure(c, b->items, sizeof(Item) * nitems)) ||
@@ -79,8 +81,62 @@ static int batch_plan(hcs_batch* b) {
     rc = hip_check(c, hipMemcpyAsync(b->items.ptr, b->plan.items.data(), sizeof(Item) * b->plan.items.size(),
                                      hipMemcpyHostToDevice, c->stream),
                    "upload items");
+  if (!rc)
+    rc = pipe_plan(c, b->pipe, (const EnvDesc*)b->envs.ptr, (const Item*)b->items.ptr, (int)b->plan.items.size());
   if (rc) return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+/* Replaces every triangle / soft item whose candidate count in one measured
+ * step exceeds split_target by up to 64 sub-items over the LBVH frontier of
+ * A at depth ceil(log2(count / split_target)).  The frontier's subtrees hold
+ * every leaf exactly once, so the candidate set is unchanged. */
+static int batch_split(hcs_batch* b) {
+  Ctx* c = b->c;
+  const int n = (int)b->plan.items.size();
+  if (n == 0 || b->split_target <= 0) return HCS_OK;
+  batch_enqueue(b, batch_args(b));
+  std::vector<int> nc(n);
+  int rc = hip_check(c, hipGetLastError(), "measure step");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(nc.data(), b->pipe.ncand.ptr, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                            c->stream), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc) return rc;
+  std::vector<Item> out;
+  out.reserve(n);
+  bool any = false;
+  for (int e = 0; e < b->npairs; ++e) {
+    EnvDesc& E = b->plan.envs[e];
+    const int first = E.item_first, cnt = E.item_count;
+    E.item_first = (int)out.size();
+    for (int i = first; i < first + cnt; ++i) {
+      const Item it = b->plan.items[i];
+      if (E.kind == KIND_TET || E.ntA <= 1 || it.root != 0 || nc[i] <= b->split_target) {
+        out.push_back(it);
+        continue;
+      }
+      int d = 0;
+      while (d < 6 && ((int64_t)b->split_target << d) < nc[i]) ++d;
+      Mesh& A = c->meshes[b->geoms[2 * e]];
+      if ((rc = mesh_frontier(c, A))) return rc;
+      for (int32_t r : A.frontier[d]) out.push_back(Item{it.env, it.first, it.count, r});
+      any = true;
+    }
+    E.item_count = (int)out.size() - E.item_first;
+  }
+  if (!any) return HCS_OK;
+  b->plan.items.swap(out);
+  return batch_upload(b);
+}
+
+/* (Re)builds the plan for the current shape and uploads it; with a state
+ * set, heavy items are split from one measured step. */
+static int batch_plan(hcs_batch* b) {
+  batch_free_graph(b);
+  int rc = build_plan(b->c, b->npairs, b->geoms.data(), &b->opts, b->qpl, b->tet_max, b->plan);
+  if (!rc) rc = batch_upload(b);
+  if (!rc && b->has_state) rc = batch_split(b);
+  return rc;
 }
 
 static int batch_capture(hcs_batch* b) {
@@ -140,6 +196,7 @@ int hcs_batch_create(hcs_ctx* h, int32_t npairs, const int32_t* geoms, const hcs
       s[0] = s[4] = s[8] = s[12] = s[16] = s[20] = 1.0;
     }
     rc = hcs_batch_set_state(b, zero.data());
+    b->has_state = false;
   }
   if (!rc) rc = batch_capture(b);
   if (rc) {
@@ -169,6 +226,7 @@ int hcs_batch_set_state(hcs_batch* b, const double* state) {
                                        hipMemcpyHostToDevice, c->stream),
                      "upload state");
   if (rc) return rc;
+  b->has_state = true;
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
@@ -206,13 +264,14 @@ int hcs_batch_counts(hcs_batch* b, int64_t* counts) {
 
 int hcs_batch_config(hcs_batch* b, int32_t* cfg) {
   if (!b || !cfg) return HCS_ERR_ARG;
-  cfg[0] = b->blocks_per_cu, cfg[1] = b->qpl, cfg[2] = b->tet_max;
+  cfg[0] = b->blocks_per_cu, cfg[1] = b->qpl, cfg[2] = b->tet_max, cfg[3] = b->split_target;
   return HCS_OK;
 }
 
 int hcs_batch_set_config(hcs_batch* b, const int32_t* cfg) {
-  if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0) return HCS_ERR_ARG;
-  b->blocks_per_cu = cfg[0], b->qpl = cfg[1], b->tet_max = cfg[2];
+  if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0 || cfg[3] < 0)
+    return HCS_ERR_ARG;
+  b->blocks_per_cu = cfg[0], b->qpl = cfg[1], b->tet_max = cfg[2], b->split_target = cfg[3];
   int rc = batch_plan(b);
   if (!rc) rc = batch_capture(b);
   return rc;
@@ -275,6 +334,9 @@ int hcs_batch_item_stats(hcs_batch* b, int64_t* out, int64_t capacity, int64_t* 
   DBuf d;
   int rc = ensure(c, d, sizeof(unsigned long long) * 4 * std::max(1, n));
   if (rc) return rc;
+  rc = hip_check(c, hipMemsetAsync(d.ptr, 0, sizeof(unsigned long long) * 4 * std::max(1, n), c->stream),
+                 "memset");
+  if (rc) return rc;
   QueryArgs a = batch_args(b);
   a.item_stats = (unsigned long long*)d.ptr;
   batch_enqueue(b, a);
@@ -308,11 +370,7 @@ int hcs_batch_kernel_times(hcs_batch* b, int32_t reps, double* ms) {
   for (int r = 0; r < reps && !rc; ++r) {
     QueryArgs a = batch_args(b);
     Pipe& P = b->pipe;
-    launch_fill_u32(s, (unsigned*)P.counter.ptr, 0u, 1);
-    launch_fill_u32(s, (unsigned*)P.surv_cnt.ptr, 0u, 2);
-    launch_fill_u32(s, (unsigned*)P.err.ptr, 0u, 1);
-    launch_fill_u32(s, (unsigned*)P.nsurv.ptr + a.nitems, 0u, 1);
-    launch_prep(s, a.envs, a.state, (double*)P.xf.ptr, b->npairs);
+    pipe_front(c, P, a, b->npairs);
     (void)hipEventRecord(ev[0], s);
     launch_cand(s, a, grid_of(b));
     (void)hipEventRecord(ev[1], s);

@@ -279,21 +279,126 @@ __global__ void k_prep(const EnvDesc* __restrict__ envs, const double* __restric
   for (int i = 0; i < 12; ++i) xf[12 * (size_t)e + i] = X[i];
 }
 
+/* Static boxes of one item per wave: [0..5] its query elements in their own
+ * frame (tets of A in A's frame; triangles or tets of B in B's frame), then
+ * [6..11] the target (B's vertex AABB in B's frame for tet queries, the
+ * item's subtree of A's LBVH otherwise). */
+__global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
+                                                   int nitems, double* __restrict__ ibox) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  if (i >= nitems) return;
+  const Item it = items[i];
+  const EnvDesc& E = envs[it.env];
+  const double inf = __builtin_inf();
+  double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
+  if (lane < it.count) {
+    if (E.kind == KIND_TET) {
+      const double* b = E.boxA + 6 * (size_t)E.leafA[it.first + lane];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lo[c] = b[c], hi[c] = b[3 + c];
+    } else {
+      const int nvq = E.kind == KIND_SOFT ? 4 : 3;
+      const int32_t* f = E.FB + (size_t)nvq * (it.first + lane);
+      for (int k = 0; k < nvq; ++k) {
+        const double* v = E.VB + 3 * (size_t)f[k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) lo[c] = fmin(lo[c], v[c]), hi[c] = fmax(hi[c], v[c]);
+      }
+    }
+  }
+  double t[6] = {inf, inf, inf, -inf, -inf, -inf};
+  if (E.kind == KIND_TET) {
+    for (int v = lane; v < E.nvB; v += 64) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) t[c] = fmin(t[c], E.VB[3 * (size_t)v + c]), t[3 + c] = fmax(t[3 + c], E.VB[3 * (size_t)v + c]);
+    }
+  } else if (E.ntA == 1 || it.root < 0) {
+    const double* b = E.boxA + 6 * (size_t)(E.ntA == 1 ? 0 : ~it.root);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) t[c] = b[c];
+  } else {
+    const Node& N = E.nodesA[it.root];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t[c] = fmin(N.b[c], N.b[6 + c]), t[3 + c] = fmax(N.b[3 + c], N.b[9 + c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    lo[c] = wave_min(lo[c]), hi[c] = wave_max(hi[c]);
+    t[c] = wave_min(t[c]), t[3 + c] = wave_max(t[3 + c]);
+  }
+  if (lane == 0) {
+    double* o = ibox + 12 * (size_t)i;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = lo[c], o[3 + c] = hi[c], o[6 + c] = t[c], o[9 + c] = t[3 + c];
+  }
+}
+
+/* Per step, one thread per item: keep the items whose static boxes may
+ * overlap under this step's pose (wave-aggregated append to the live list);
+ * a rejected item gets zero counts here. */
+__global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
+                                              int nitems, const double* __restrict__ ibox,
+                                              const double* __restrict__ xf, int* __restrict__ live,
+                                              unsigned* __restrict__ live_cnt, int* __restrict__ ncand,
+                                              unsigned* __restrict__ nsurv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool keep = false;
+  if (i < nitems) {
+    const Item it = items[i];
+    const double* q = ibox + 12 * (size_t)i;
+    const bool tet = envs[it.env].kind == KIND_TET;
+    keep = boxes_may_overlap(xf + 12 * (size_t)it.env, tet ? q + 6 : q, tet ? q : q + 6);
+    if (!keep) ncand[i] = 0, nsurv[i] = 0;
+  }
+  const unsigned long long m = __ballot(keep);
+  unsigned base = 0;
+  if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(live_cnt, (unsigned)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (keep) live[base + lane_prefix(m)] = i;
+}
+
+/* LBVH frontier at depths 0..6 (one thread): out[65 d] = count, then the
+ * node codes (a leaf reached early stays in every deeper level). */
+__global__ void k_frontier(const Node* __restrict__ nodes, int nt, int32_t* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int32_t cur[64], nxt[64];
+  int nc = 1;
+  cur[0] = nt > 1 ? 0 : ~0;
+  for (int d = 0; d < 7; ++d) {
+    out[65 * d] = nc;
+    for (int k = 0; k < nc; ++k) out[65 * d + 1 + k] = cur[k];
+    if (d == 6) break;
+    int nn = 0;
+    for (int k = 0; k < nc; ++k) {
+      if (cur[k] < 0) {
+        nxt[nn++] = cur[k];
+      } else {
+        nxt[nn++] = nodes[cur[k]].c[0];
+        nxt[nn++] = nodes[cur[k]].c[1];
+      }
+    }
+    for (int k = 0; k < nn; ++k) cur[k] = nxt[k];
+    nc = nn;
+  }
+}
+
 /* Candidate generation: one wave per item; persistent when a work-queue
  * counter is given (the counter is reset by a fill kernel in the same
  * graph). */
+template <bool KEYS>
 __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
+  const unsigned n = A.live ? *A.live_cnt : (unsigned)A.nitems;
   if (A.counter == nullptr) {
-    if ((int)blockIdx.x < A.nitems) run_cand(L, A, blockIdx.x);
+    if (blockIdx.x < n) run_cand<KEYS>(L, A, A.live ? A.live[blockIdx.x] : (int)blockIdx.x);
     return;
   }
   for (;;) {
     unsigned v = 0;
     if (threadIdx.x == 0) v = atomicAdd(A.counter, 1u);
-    const int it = (int)__shfl(v, 0, 64);
-    if (it >= A.nitems) break;
-    run_cand(L, A, it);
+    v = __shfl(v, 0, 64);
+    if (v >= n) break;
+    run_cand<KEYS>(L, A, A.live ? A.live[v] : (int)v);
   }
 }
 
@@ -396,22 +501,41 @@ QueryArgs pipe_args(const Pipe& p, const EnvDesc* envs, const double* state, con
   a.surv_cap = p.cap;
   a.item_ncand = (int*)p.ncand.ptr;
   a.item_nsurv = (unsigned*)p.nsurv.ptr;
+  a.live = p.live.ptr ? (const int*)p.live.ptr : nullptr;
+  a.live_cnt = p.live.ptr ? (const unsigned*)p.counter.ptr + 1 : nullptr;
   return a;
 }
 
-void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid, int clip_grid, int ndir,
-                  const double* dstar_dE, const double* invE, bool cand_only, int dir0) {
+void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs) {
   hipStream_t s = c->stream;
-  launch_fill_u32(s, (unsigned*)p.counter.ptr, 0u, 1);
+  launch_fill_u32(s, (unsigned*)p.counter.ptr, 0u, 2);
   launch_fill_u32(s, (unsigned*)p.surv_cnt.ptr, 0u, 2);
   launch_fill_u32(s, (unsigned*)p.err.ptr, 0u, 1);
   launch_fill_u32(s, (unsigned*)p.nsurv.ptr + a.nitems, 0u, 1);
   launch_prep(s, a.envs, a.state, (double*)p.xf.ptr, npairs);
-  if (a.nitems > 0 && cand_grid > 0) hipLaunchKernelGGL(k_cand, dim3(cand_grid), dim3(64), 0, s, a);
-  if (cand_only) return;
+  if (a.nitems > 0 && a.live)
+    hipLaunchKernelGGL(k_cull, dim3((a.nitems + 255) / 256), dim3(256), 0, s, a.envs, a.items, a.nitems,
+                       (const double*)p.ibox.ptr, (const double*)p.xf.ptr, (int*)p.live.ptr,
+                       (unsigned*)p.counter.ptr + 1, a.item_ncand, a.item_nsurv);
+}
+
+void launch_cand(hipStream_t s, const QueryArgs& a, int grid) {
+  if (a.nitems <= 0 || grid <= 0) return;
+  if (a.cand_key)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<true>), dim3(grid), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<false>), dim3(grid), dim3(64), 0, s, a);
+}
+
+/* Scan of the per-item survivor counts, [event e_clip], clip (double, or
+ * Dual over ndir directions), [event e_reduce], per-pair reduction. */
+static void pipe_back(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, int ndir,
+                      const double* dstar_dE, const double* invE, int dir0, hipEvent_t e_clip, hipEvent_t e_reduce) {
+  hipStream_t s = c->stream;
   size_t bytes = p.scan_bytes;
   hipcub::DeviceScan::ExclusiveSum(p.scan_tmp.ptr, bytes, (const unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr,
                                    a.nitems + 1, s);
+  if (e_clip) (void)hipEventRecord(e_clip, s);
   ClipArgs C{(const unsigned*)p.off.ptr, dstar_dE, invE, p.cap};
   const int g = std::max(1, clip_grid);
   if (ndir <= 0)
@@ -420,6 +544,7 @@ void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid
   else
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<Dual>), dim3(g, ndir), dim3(256), 0, s, a, C, dir0,
                        (Contrib*)p.contrib.ptr, (int64_t)p.cap);
+  if (e_reduce) (void)hipEventRecord(e_reduce, s);
   if (npairs > 0)
     hipLaunchKernelGGL(k_env_reduce, dim3(npairs, std::max(1, ndir)), dim3(64), 0, s, a.envs,
                        (const unsigned*)p.off.ptr, (const int*)p.ncand.ptr, (const unsigned long long*)p.surv_cnt.ptr,
@@ -427,33 +552,49 @@ void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid
                        (PairOut*)p.out.ptr, npairs);
 }
 
-void launch_cand(hipStream_t s, const QueryArgs& a, int grid) {
-  if (a.nitems > 0 && grid > 0) hipLaunchKernelGGL(k_cand, dim3(grid), dim3(64), 0, s, a);
+void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid, int clip_grid, int ndir,
+                  const double* dstar_dE, const double* invE, bool cand_only, int dir0) {
+  pipe_front(c, p, a, npairs);
+  launch_cand(c->stream, a, cand_grid);
+  if (cand_only) return;
+  pipe_back(c, p, a, npairs, clip_grid, ndir, dstar_dE, invE, dir0, nullptr, nullptr);
 }
 
-/* scan, event e_clip, k_clip, event e_reduce, k_env_reduce (forward). */
 void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, hipEvent_t e_clip,
                         hipEvent_t e_reduce) {
-  hipStream_t s = c->stream;
-  size_t bytes = p.scan_bytes;
-  hipcub::DeviceScan::ExclusiveSum(p.scan_tmp.ptr, bytes, (const unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr,
-                                   a.nitems + 1, s);
-  (void)hipEventRecord(e_clip, s);
-  ClipArgs C{(const unsigned*)p.off.ptr, nullptr, nullptr, p.cap};
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double>), dim3(std::max(1, clip_grid)), dim3(256), 0, s, a, C, -1,
-                     (Contrib*)p.contrib.ptr, (int64_t)0);
-  (void)hipEventRecord(e_reduce, s);
-  if (npairs > 0)
-    hipLaunchKernelGGL(k_env_reduce, dim3(npairs, 1), dim3(64), 0, s, a.envs, (const unsigned*)p.off.ptr,
-                       (const int*)p.ncand.ptr, (const unsigned long long*)p.surv_cnt.ptr, p.cap,
-                       (const Contrib*)p.contrib.ptr, (int64_t)p.cap, (const int*)p.err.ptr, (PairOut*)p.out.ptr,
-                       npairs);
+  pipe_back(c, p, a, npairs, clip_grid, 0, nullptr, nullptr, 0, e_clip, e_reduce);
+}
+
+int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, int nitems) {
+  int rc;
+  const size_t ni = (size_t)std::max(1, nitems);
+  if ((rc = ensure(c, p.ibox, sizeof(double) * 12 * ni)) || (rc = ensure(c, p.live, sizeof(int) * ni))) return rc;
+  if (nitems > 0) hipLaunchKernelGGL(k_item_boxes, dim3(nitems), dim3(64), 0, c->stream, envs, items, nitems,
+                                     (double*)p.ibox.ptr);
+  return hip_check(c, hipGetLastError(), "item boxes");
 }
 
 void pipe_free(Pipe& p) {
   for (DBuf* b : {&p.counter, &p.surv, &p.surv_cnt, &p.ncand, &p.nsurv, &p.off, &p.scan_tmp, &p.contrib, &p.out, &p.err,
-                  &p.xf})
+                  &p.xf, &p.ibox, &p.live})
     if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
+}
+
+int mesh_frontier(Ctx* c, Mesh& m) {
+  if (!m.frontier.empty()) return HCS_OK;
+  DBuf d;
+  int rc = ensure(c, d, sizeof(int32_t) * 7 * 65);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(64), 0, c->stream, (const Node*)m.nodes, m.nelem, (int32_t*)d.ptr);
+  std::vector<int32_t> h(7 * 65);
+  rc = hip_check(c, hipGetLastError(), "frontier");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(h.data(), d.ptr, h.size() * 4, hipMemcpyDeviceToHost, c->stream), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  (void)hipFree(d.ptr);
+  if (rc) return rc;
+  m.frontier.assign(7, {});
+  for (int dd = 0; dd < 7; ++dd) m.frontier[dd].assign(h.begin() + 65 * dd + 1, h.begin() + 65 * dd + 1 + h[65 * dd]);
+  return HCS_OK;
 }
 
 /* ===================================================================
@@ -609,6 +750,7 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
     plan.pair_kind[e] = E.kind;
     E.item_first = (int)plan.items.size();
     for (int q = 0; q < nq; q += chunk) plan.items.push_back(Item{e, q, std::min(chunk, nq - q), 0});
+    /* (root 0: the whole LBVH of A) */
     E.item_count = (int)plan.items.size() - E.item_first;
   }
   return HCS_OK;
@@ -812,6 +954,7 @@ static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_o
   const int nitems = (int)plan.items.size();
   Pipe& P = c->pipe;
   if ((rc = pipe_reserve(c, P, nitems, npairs, std::max<int64_t>(P.cap, 4096), 0))) return rc;
+  if ((rc = pipe_plan(c, P, (const EnvDesc*)c->d_envs.ptr, (const Item*)c->d_items.ptr, nitems))) return rc;
   if ((rc = ensure(c, c->d_face_cnt, 64)) || (rc = ensure(c, c->d_poly_cnt, 64)) || (rc = ensure(c, c->d_cand_cnt, 64)))
     return rc;
   const bool keep = opts && opts->keep_polygons && !keys;

@@ -26,6 +26,9 @@ struct Mesh {
   Node* nodes = nullptr;
   int32_t* leaf = nullptr; /* tets in Morton (BVH leaf) order */
   double* nrm = nullptr;
+  /* host copy of the LBVH frontier at depths 0..6 (node codes as in
+   * Node::c), filled on first use by mesh_frontier */
+  std::vector<std::vector<int32_t>> frontier;
 };
 
 /* Grow-only device buffer. */
@@ -37,7 +40,8 @@ struct DBuf {
 /* The per-step pipeline (hcs_engine.hip): k_prep, k_cand, scan of
  * per-item survivor counts, k_clip, k_env_reduce. */
 struct Pipe {
-  DBuf counter, surv, surv_cnt, ncand, nsurv, off, scan_tmp, contrib, out, err, xf;
+  DBuf counter /* [0] work queue, [1] live items */, surv, surv_cnt, ncand, nsurv, off, scan_tmp, contrib, out, err,
+      xf, ibox, live;
   int64_t cap = 0;
   size_t scan_bytes = 0;
 };
@@ -89,6 +93,13 @@ QueryArgs pipe_args(const Pipe& p, const EnvDesc* envs, const double* state, con
  * dir0+ndir-1 (contributions and outputs indexed by direction - dir0). */
 void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid, int clip_grid, int ndir,
                   const double* dstar_dE, const double* invE, bool cand_only, int dir0);
+/* Static boxes of the current items (once per plan, after the items and
+ * env descriptors are on the device). */
+int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, int nitems);
+/* Counter resets, pose composition and the item cull of one step. */
+void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs);
+/* LBVH frontier of a soft mesh (node codes per depth 0..6). */
+int mesh_frontier(Ctx* c, Mesh& m);
 void pipe_free(Pipe& p);
 void launch_cand(hipStream_t s, const QueryArgs& a, int grid);
 void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, hipEvent_t e_clip,

@@ -12,9 +12,12 @@
  *               leaf  nt int32 tets in Morton order
  *               T     nt x 4 int32, V nv x 3, p nv
  *   rigid mesh: V nv x 3, F ntri x 3 int32, nrm ntri x 3 (unit normals)
+ * Per plan:    items (16 B), static item boxes (12 doubles: the query
+ *              elements' box in their own frame, then the target's box)
  * Per step:    state  npairs x 36 (X_WA, X_WB, V_WA, V_WB), xf npairs x 12
- *              (X_AB = X_WA^-1 X_WB), items, per-item partial wrenches,
- *              per-pair outputs.
+ *              (X_AB = X_WA^-1 X_WB), live item list, survivors (16 B),
+ *              per-item counts and scan, contributions (80 B), per-pair
+ *              outputs.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -54,8 +57,12 @@ struct EnvDesc {
   int32_t pad[2];
 };
 
+/* A work item: queries first .. first+count-1 of pair env.  Triangle and
+ * soft queries walk A's LBVH from node root (0: the whole tree; < 0: the
+ * single leaf ~root), so a heavy item can be split into sub-items over
+ * disjoint subtrees. */
 struct Item {
-  int32_t env, first, count, pad;
+  int32_t env, first, count, root;
 };
 
 struct PairOut {
@@ -93,6 +100,9 @@ struct QueryArgs {
   int64_t surv_cap;
   int* item_ncand;
   unsigned* item_nsurv;
+  /* items whose static boxes overlap this step (nullptr: all items) */
+  const int* live;
+  const unsigned* live_cnt;
 };
 
 /* Inputs of the clip kernel beyond QueryArgs. */

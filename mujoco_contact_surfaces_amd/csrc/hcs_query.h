@@ -488,6 +488,35 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b)
   return above != 0 && above != 4;
 }
 
+/* Whether an item can have a candidate: box P (B's frame) placed in A's
+ * frame by X = [R | t] (y = R x + t) and bounded there by the axis-aligned
+ * box of the rotated box, against box Q (A's frame).  The candidate test
+ * compares axis-aligned boxes in A's frame, and every element box of P maps
+ * into that bound, so this test alone is exact for it (a separating axis of
+ * B's frame would not be: an element's A-frame box leaves P's B-frame box).
+ * The tolerance (1e-9 of the coordinates involved) is many orders above
+ * the rounding of the transformed element boxes. */
+__device__ inline bool boxes_may_overlap(const double* X, const double* P, const double* Q) {
+  double cp[3], hp[3], cq[3], hq[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    cp[c] = 0.5 * (P[c] + P[3 + c]), hp[c] = 0.5 * (P[3 + c] - P[c]);
+    cq[c] = 0.5 * (Q[c] + Q[3 + c]), hq[c] = 0.5 * (Q[3 + c] - Q[c]);
+  }
+  double mag = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    mag = fmax(mag, fmax(fabs(X[9 + c]), fmax(fabs(cp[c]) + hp[c], fabs(cq[c]) + hq[c])));
+  const double tol = 1e-9 * mag + 1e-300;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double c1 = ((X[3 * i] * cp[0] + X[3 * i + 1] * cp[1]) + X[3 * i + 2] * cp[2]) + X[9 + i];
+    const double h1 = (fabs(X[3 * i]) * hp[0] + fabs(X[3 * i + 1]) * hp[1]) + fabs(X[3 * i + 2]) * hp[2];
+    if (fabs(c1 - cq[i]) > (h1 + hq[i]) + tol) return false;
+  }
+  return true;
+}
+
 /* ===================================================================
  * k_cand: candidate generation, exact cheap tests, survivor compaction
  * =================================================================== */
@@ -508,7 +537,9 @@ struct CandLds {
  * tests on every 64 queued candidates, and append survivors to HBM in
  * groups of up to 64 with a per-item sequence number, so the clip kernel can
  * place each contribution at item_off[item] + seq whatever order the groups
- * land in. */
+ * land in.  KEYS: write the canonical candidate keys instead (no tests, no
+ * survivors). */
+template <bool KEYS>
 __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
   const int lane = threadIdx.x & 63;
   const unsigned long long t_begin = A.item_stats ? clock64() : 0ull;
@@ -566,13 +597,15 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = lo[c], L.box[6 * lane + 3 + c] = hi[c];
     }
-    if (E.ntA == 1) {
-      const bool hit = valid && box_overlap(E.boxA, lo, hi);
+    if (E.ntA == 1 || item.root < 0) {
+      /* a one-tet mesh, or a sub-item whose subtree is a single leaf */
+      const int leaf = E.ntA == 1 ? 0 : ~item.root;
+      const bool hit = valid && box_overlap(E.boxA + 6 * (size_t)leaf, lo, hi);
       const unsigned long long m = __ballot(hit);
-      if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(0, b);
+      if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(leaf, b);
       q1n += __popcll(m);
     } else if (valid) {
-      stk[0] = 0;
+      stk[0] = item.root;
       sp = 1;
     }
   }
@@ -718,7 +751,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
         if (lane < take) {
           c = L.q1[base + lane];
           ++nc;
-          if (A.cand_key) {
+          if (KEYS) {
             unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
             if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
           } else {
