@@ -237,6 +237,17 @@ def test_batched_matches_plain_and_oracle(engine):
     np.testing.assert_array_equal(W3, W2)
     times = b.kernel_times(2)
     assert times["cand"] > 0 and times["clip"] > 0
+    # heavy items split over LBVH subtrees (a tiny split target splits
+    # nearly every triangle item): same candidate set, same wrenches
+    say("batched split")
+    for tet_max in (0, 64):
+        b.set_config(2, 1, tet_max, 8)
+        b.step()
+        W4, cnt4 = b.get()
+        np.testing.assert_array_equal(cnt4, cnt)
+        for i in range(len(scs)):
+            assert_wrench_close(W4[i], W[i], rtol=1e-12)
+    assert b.config()["split_target"] == 8
     b.close()
 
 
