@@ -87,46 +87,61 @@ static int batch_upload(hcs_batch* b) {
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
 
-/* Replaces every triangle / soft item whose candidate count in one measured
- * step exceeds split_target by up to 64 sub-items over the LBVH frontier of
- * A at depth ceil(log2(count / split_target)).  The frontier's subtrees hold
- * every leaf exactly once, so the candidate set is unchanged. */
+/* Splits every triangle / soft item whose candidate count in a measured
+ * step exceeds split_target into sub-items over the LBVH frontier below its
+ * root, at depth ceil(log2(count / split_target)) <= 6, and repeats on the
+ * new plan (at most 4 rounds) while some item still exceeds it.  The
+ * frontier's subtrees hold each leaf exactly once, so the candidate set is
+ * unchanged. */
 static int batch_split(hcs_batch* b) {
   Ctx* c = b->c;
-  const int n = (int)b->plan.items.size();
-  if (n == 0 || b->split_target <= 0) return HCS_OK;
-  batch_enqueue(b, batch_args(b));
-  std::vector<int> nc(n);
-  int rc = hip_check(c, hipGetLastError(), "measure step");
-  if (!rc) rc = hip_check(c, hipMemcpyAsync(nc.data(), b->pipe.ncand.ptr, sizeof(int) * n, hipMemcpyDeviceToHost,
-                                            c->stream), "dl");
-  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
-  if (rc) return rc;
-  std::vector<Item> out;
-  out.reserve(n);
-  bool any = false;
-  for (int e = 0; e < b->npairs; ++e) {
-    EnvDesc& E = b->plan.envs[e];
-    const int first = E.item_first, cnt = E.item_count;
-    E.item_first = (int)out.size();
-    for (int i = first; i < first + cnt; ++i) {
-      const Item it = b->plan.items[i];
-      if (E.kind == KIND_TET || E.ntA <= 1 || it.root != 0 || nc[i] <= b->split_target) {
-        out.push_back(it);
-        continue;
-      }
-      int d = 0;
+  if (b->split_target <= 0) return HCS_OK;
+  for (int round = 0; round < 4; ++round) {
+    const int n = (int)b->plan.items.size();
+    if (n == 0) return HCS_OK;
+    batch_enqueue(b, batch_args(b));
+    std::vector<int> nc(n);
+    int rc = hip_check(c, hipGetLastError(), "measure step");
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(nc.data(), b->pipe.ncand.ptr, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                       c->stream), "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    if (rc) return rc;
+    std::vector<int> req, depth, slot(n, -1);
+    for (int i = 0; i < n; ++i) {
+      const Item& it = b->plan.items[i];
+      const EnvDesc& E = b->plan.envs[it.env];
+      if (E.kind == KIND_TET || E.ntA <= 1 || it.root < 0 || nc[i] <= b->split_target) continue;
+      int d = 1;
       while (d < 6 && ((int64_t)b->split_target << d) < nc[i]) ++d;
-      Mesh& A = c->meshes[b->geoms[2 * e]];
-      if ((rc = mesh_frontier(c, A))) return rc;
-      for (int32_t r : A.frontier[d]) out.push_back(Item{it.env, it.first, it.count, r});
-      any = true;
+      slot[i] = (int)req.size();
+      req.push_back(i);
+      depth.push_back(d);
     }
-    E.item_count = (int)out.size() - E.item_first;
+    if (req.empty()) return HCS_OK;
+    std::vector<int32_t> roots;
+    if ((rc = split_roots(c, (const EnvDesc*)b->envs.ptr, (const Item*)b->items.ptr, req, depth, roots))) return rc;
+    std::vector<Item> out;
+    out.reserve(n + 64 * req.size());
+    for (int e = 0; e < b->npairs; ++e) {
+      EnvDesc& E = b->plan.envs[e];
+      const int first = E.item_first, cnt = E.item_count;
+      E.item_first = (int)out.size();
+      for (int i = first; i < first + cnt; ++i) {
+        const Item it = b->plan.items[i];
+        if (slot[i] < 0) {
+          out.push_back(it);
+          continue;
+        }
+        const int32_t* r = &roots[65 * (size_t)slot[i]];
+        for (int k = 0; k < r[0]; ++k) out.push_back(Item{it.env, it.first, it.count, r[1 + k]});
+      }
+      E.item_count = (int)out.size() - E.item_first;
+    }
+    b->plan.items.swap(out);
+    if ((rc = batch_upload(b))) return rc;
   }
-  if (!any) return HCS_OK;
-  b->plan.items.swap(out);
-  return batch_upload(b);
+  return HCS_OK;
 }
 
 /* (Re)builds the plan for the current shape and uploads it; with a state

@@ -357,17 +357,22 @@ __global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, 
   if (keep) live[base + lane_prefix(m)] = i;
 }
 
-/* LBVH frontier at depths 0..6 (one thread): out[65 d] = count, then the
- * node codes (a leaf reached early stays in every deeper level). */
-__global__ void k_frontier(const Node* __restrict__ nodes, int nt, int32_t* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+/* Sub-roots for splitting (one thread per request): the frontier of the LBVH
+ * of the request's item below the item's root at depth[r] <= 6; out[65 r] =
+ * count, then node codes (a leaf reached early stays a leaf).  The
+ * frontier's subtrees hold each leaf of the root's subtree exactly once. */
+__global__ void k_split_roots(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
+                              const int* __restrict__ req, const int* __restrict__ depth, int nreq,
+                              int32_t* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nreq) return;
+  const Item it = items[req[r]];
+  const Node* nodes = envs[it.env].nodesA;
   int32_t cur[64], nxt[64];
   int nc = 1;
-  cur[0] = nt > 1 ? 0 : ~0;
-  for (int d = 0; d < 7; ++d) {
-    out[65 * d] = nc;
-    for (int k = 0; k < nc; ++k) out[65 * d + 1 + k] = cur[k];
-    if (d == 6) break;
+  cur[0] = it.root;
+  const int dmax = depth[r] < 6 ? depth[r] : 6;
+  for (int d = 0; d < dmax; ++d) {
     int nn = 0;
     for (int k = 0; k < nc; ++k) {
       if (cur[k] < 0) {
@@ -380,6 +385,8 @@ __global__ void k_frontier(const Node* __restrict__ nodes, int nt, int32_t* __re
     for (int k = 0; k < nn; ++k) cur[k] = nxt[k];
     nc = nn;
   }
+  out[65 * (size_t)r] = nc;
+  for (int k = 0; k < nc; ++k) out[65 * (size_t)r + 1 + k] = cur[k];
 }
 
 /* Candidate generation: one wave per item; persistent when a work-queue
@@ -389,17 +396,19 @@ template <bool KEYS>
 __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
   const unsigned n = A.live ? *A.live_cnt : (unsigned)A.nitems;
+  unsigned long long s_next = 0, s_end = 0; /* this wave's survivor slots */
   if (A.counter == nullptr) {
-    if (blockIdx.x < n) run_cand<KEYS>(L, A, A.live ? A.live[blockIdx.x] : (int)blockIdx.x);
-    return;
+    if (blockIdx.x < n) run_cand<KEYS>(L, A, A.live ? A.live[blockIdx.x] : (int)blockIdx.x, s_next, s_end);
+  } else {
+    for (;;) {
+      unsigned v = 0;
+      if (threadIdx.x == 0) v = atomicAdd(A.counter, 1u);
+      v = __shfl(v, 0, 64);
+      if (v >= n) break;
+      run_cand<KEYS>(L, A, A.live ? A.live[v] : (int)v, s_next, s_end);
+    }
   }
-  for (;;) {
-    unsigned v = 0;
-    if (threadIdx.x == 0) v = atomicAdd(A.counter, 1u);
-    v = __shfl(v, 0, 64);
-    if (v >= n) break;
-    run_cand<KEYS>(L, A, A.live ? A.live[v] : (int)v);
-  }
+  fill_sentinels(A, s_next, s_end);
 }
 
 /* Clip, faces and force of every survivor (grid-stride); grid.y selects the
@@ -580,21 +589,33 @@ void pipe_free(Pipe& p) {
     if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
 }
 
-int mesh_frontier(Ctx* c, Mesh& m) {
-  if (!m.frontier.empty()) return HCS_OK;
-  DBuf d;
-  int rc = ensure(c, d, sizeof(int32_t) * 7 * 65);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(64), 0, c->stream, (const Node*)m.nodes, m.nelem, (int32_t*)d.ptr);
-  std::vector<int32_t> h(7 * 65);
-  rc = hip_check(c, hipGetLastError(), "frontier");
-  if (!rc) rc = hip_check(c, hipMemcpyAsync(h.data(), d.ptr, h.size() * 4, hipMemcpyDeviceToHost, c->stream), "dl");
+int split_roots(Ctx* c, const EnvDesc* envs, const Item* items, const std::vector<int>& req,
+                const std::vector<int>& depth, std::vector<int32_t>& out) {
+  const int n = (int)req.size();
+  out.assign(65 * (size_t)n, 0);
+  if (n == 0) return HCS_OK;
+  DBuf dreq, ddep, dout;
+  auto done = [&](int rc) {
+    for (DBuf* b : {&dreq, &ddep, &dout})
+      if (b->ptr) (void)hipFree(b->ptr);
+    return rc;
+  };
+  int rc;
+  if ((rc = ensure(c, dreq, sizeof(int) * n)) || (rc = ensure(c, ddep, sizeof(int) * n)) ||
+      (rc = ensure(c, dout, sizeof(int32_t) * out.size())))
+    return done(rc);
+  rc = hip_check(c, hipMemcpyAsync(dreq.ptr, req.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream), "ul");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(ddep.ptr, depth.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream), "ul");
+  if (rc) return done(rc);
+  hipLaunchKernelGGL(k_split_roots, dim3((n + 63) / 64), dim3(64), 0, c->stream, envs, items, (const int*)dreq.ptr,
+                     (const int*)ddep.ptr, n, (int32_t*)dout.ptr);
+  rc = hip_check(c, hipGetLastError(), "split roots");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(out.data(), dout.ptr, sizeof(int32_t) * out.size(), hipMemcpyDeviceToHost,
+                                     c->stream), "dl");
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
-  (void)hipFree(d.ptr);
-  if (rc) return rc;
-  m.frontier.assign(7, {});
-  for (int dd = 0; dd < 7; ++dd) m.frontier[dd].assign(h.begin() + 65 * dd + 1, h.begin() + 65 * dd + 1 + h[65 * dd]);
-  return HCS_OK;
+  return done(rc);
 }
 
 /* ===================================================================

@@ -26,9 +26,6 @@ struct Mesh {
   Node* nodes = nullptr;
   int32_t* leaf = nullptr; /* tets in Morton (BVH leaf) order */
   double* nrm = nullptr;
-  /* host copy of the LBVH frontier at depths 0..6 (node codes as in
-   * Node::c), filled on first use by mesh_frontier */
-  std::vector<std::vector<int32_t>> frontier;
 };
 
 /* Grow-only device buffer. */
@@ -98,8 +95,11 @@ void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid
 int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, int nitems);
 /* Counter resets, pose composition and the item cull of one step. */
 void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs);
-/* LBVH frontier of a soft mesh (node codes per depth 0..6). */
-int mesh_frontier(Ctx* c, Mesh& m);
+/* For each requested item (index into the device items), the LBVH
+ * frontier below its root at the requested depth (<= 6): out[65 r] = count,
+ * then node codes. */
+int split_roots(Ctx* c, const EnvDesc* envs, const Item* items, const std::vector<int>& req,
+                const std::vector<int>& depth, std::vector<int32_t>& out);
 void pipe_free(Pipe& p);
 void launch_cand(hipStream_t s, const QueryArgs& a, int grid);
 void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, hipEvent_t e_clip,

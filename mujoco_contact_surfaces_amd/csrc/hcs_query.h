@@ -432,17 +432,16 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
  * soft-soft: the equal-pressure plane, both gradient filters and the
  * marching-tets sign test of process_candidate (same arithmetic). */
 template <class T>
-__device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b) {
+__device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b, const double* tv, const double* tn) {
   const double* r = E.recA + (size_t)HCS_TREC * a;
   if (E.kind != KIND_SOFT) {
-    const int32_t* f = E.FB + 3 * (size_t)b;
-    const double* nr = E.nrmB + 3 * (size_t)b;
-    double nS[3];
-    rot(X, nr, nS);
+    /* tv, tn: triangle b's vertices and unit normal already moved into A's
+     * frame (xform / rot with X, the same arithmetic as process_candidate) */
+    const double nS[3] = {tn[0], tn[1], tn[2]};
     if (!(((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA)) return false;
     double v[3][3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) xform(X, E.VB + 3 * (size_t)f[i], v[i]);
+    for (int i = 0; i < 3; ++i) v[i][0] = tv[3 * i], v[i][1] = tv[3 * i + 1], v[i][2] = tv[3 * i + 2];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const double* n = r + 3 * k;
@@ -528,9 +527,21 @@ struct CandLds {
   int2 q1[192];
   int2 q2[128];
   double box[64 * 6];
+  double tv[64 * 9]; /* query / tile triangles in A's frame */
+  double tn[64 * 3]; /* their unit normals in A's frame */
   int2 xfer[64];
   double X[12];
 };
+
+/* Survivor slots are reserved per wave in chunks of this many (one atomic
+ * per chunk); slots a wave leaves unused are filled with sentinels. */
+#define HCS_SURV_CHUNK 256
+
+__device__ inline void fill_sentinels(const QueryArgs& A, unsigned long long from, unsigned long long to) {
+  const int lane = threadIdx.x & 63;
+  for (unsigned long long k = from + lane; k < to; k += 64)
+    if ((int64_t)k < A.surv_cap) A.surv[k] = make_int4(-1, -1, -1, -1);
+}
 
 /* One item: produce candidates (traversal with intra-wave work stealing, or
  * rigid-triangle tiles against 64 Morton-ordered tets), run the exact cheap
@@ -540,7 +551,8 @@ struct CandLds {
  * land in.  KEYS: write the canonical candidate keys instead (no tests, no
  * survivors). */
 template <bool KEYS>
-__device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
+__device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, unsigned long long& s_next,
+                                unsigned long long& s_end) {
   const int lane = threadIdx.x & 63;
   const unsigned long long t_begin = A.item_stats ? clock64() : 0ull;
   unsigned long long steps = 0;
@@ -588,11 +600,21 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
       for (int i = 0; i < nvq; ++i) {
         double y[3];
         xform(X, E.VB + 3 * (size_t)f[i], y);
+        if (i < 3) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) L.tv[9 * lane + 3 * i + c] = y[c];
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
           hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
         }
+      }
+      if (E.kind != KIND_SOFT) {
+        double nS[3];
+        rot(X, E.nrmB + 3 * (size_t)b, nS);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = lo[c], L.box[6 * lane + 3 + c] = hi[c];
@@ -610,9 +632,11 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
     }
   }
   for (;;) {
-    /* ---- produce */
+    /* ---- produce; in tet mode the queued candidates refer to the current
+     * tile's triangles in LDS, so they are drained before the next tile */
     ++steps;
-    if (producing) {
+    const bool drain = tet_mode && producing && tmask == 0 && q1n > 0;
+    if (producing && !drain) {
       if (tet_mode) {
         if (tmask == 0) {
           /* next tile of 64 rigid triangles: boxes into LDS, keep those that
@@ -634,10 +658,15 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
                 xform(X, E.VB + 3 * (size_t)f[i], y);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
+                  L.tv[9 * lane + 3 * i + c] = y[c];
                   l3[c] = i == 0 ? y[c] : fmin(l3[c], y[c]);
                   h3[c] = i == 0 ? y[c] : fmax(h3[c], y[c]);
                 }
               }
+              double nS[3];
+              rot(X, E.nrmB + 3 * (size_t)(t0 + lane), nS);
+#pragma unroll
+              for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
 #pragma unroll
               for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = l3[c], L.box[6 * lane + 3 + c] = h3[c];
               const double ub[6] = {ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2]};
@@ -727,15 +756,22 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
      * q1 < 192 and q2 < 128 hold throughout */
     for (;;) {
       const bool do2 = q2n >= 64 || (!producing && q1n == 0 && q2n > 0);
-      const bool do1 = q1n >= 64 || (!producing && q1n > 0);
+      const bool do1 = q1n >= 64 || ((!producing || drain) && q1n > 0);
       if (!do1 && !do2) break;
       __syncthreads();
       if (do2) {
         const int take = q2n >= 64 ? 64 : q2n;
         const int base = q2n - take;
-        unsigned long long slot0 = 0;
-        if (lane == 0) slot0 = atomicAdd(A.surv_cnt, (unsigned long long)take);
-        slot0 = __shfl(slot0, 0, 64);
+        if (s_next + take > s_end) {
+          /* new chunk of survivor slots; the rest of the old one is padded */
+          fill_sentinels(A, s_next, s_end);
+          unsigned long long v = 0;
+          if (lane == 0) v = atomicAdd(A.surv_cnt, (unsigned long long)HCS_SURV_CHUNK);
+          s_next = __shfl(v, 0, 64);
+          s_end = s_next + HCS_SURV_CHUNK;
+        }
+        const unsigned long long slot0 = s_next;
+        s_next += take;
         if (lane < take) {
           const int2 c = L.q2[base + lane];
           const unsigned long long slot = slot0 + lane;
@@ -755,7 +791,8 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it) {
             unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
             if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
           } else {
-            keep = survives<double>(E, X, c.x, c.y);
+            const int slot = tet_mode ? c.y - t0 : c.y - item.first;
+            keep = survives<double>(E, X, c.x, c.y, &L.tv[9 * slot], &L.tn[3 * slot]);
           }
         }
         q1n = base;
@@ -792,6 +829,7 @@ template <class T>
 __device__ inline void clip_survivor(const QueryArgs& A, const ClipArgs& C, int64_t i, int dir, Contrib* out,
                                      const OutPtrs* o) {
   const int4 s = A.surv[i];
+  if (s.x < 0) return; /* unused slot of a wave's reserved chunk */
   const int64_t slot = (int64_t)C.item_off[s.z] + (unsigned)s.w;
   if (slot >= C.cap) return;
   const int env = A.items[s.z].env;
