@@ -125,8 +125,22 @@ def main():
     ap.add_argument("--plain-envs", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--config", default="", help="fixed launch shape: blocks_per_cu,queries_per_lane,tet_max[,split_target]")
+    ap.add_argument("--config", default="", help="fixed launch shape: blocks_per_cu,queries_per_lane,tet_max[,split_target[,queues]]")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (scale reference for the HBM "
+                         "byte counters of a profiler) and exit")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_bytes=n, copies=5)))
+        return
     if a.gpus != 1:
         print("bench.py measures one GPU only: the engine is single-device (multi-GPU sharding is design-only); "
               "run with --gpus 1", file=sys.stderr)

@@ -178,7 +178,8 @@ int hcs_batch_get(hcs_batch* batch, hcs_pair_out* out);
 int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
 /* Launch configuration: cfg = [persistent blocks per CU, queries per lane,
  * largest rigid mesh handled in tet-query mode, candidate count above which
- * a work item is split over LBVH subtrees (0: never)]; 4 int32. */
+ * a work item is split over LBVH subtrees (0: never), work queues of the
+ * candidate kernel (1, or up to 8: one per XCD)]; 5 int32. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
 int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
 /* Eager per-kernel timing with HIP events on hcs_stream, averaged over

@@ -72,7 +72,7 @@ static int tangent_pass(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_ste
   Pipe& P = c->pipe;
   for (int attempt = 0; attempt < 3 && !rc; ++attempt) {
     rc = pipe_reserve(c, P, nitems, npairs, std::max<int64_t>(P.cap, 4096), ndir);
-    if (!rc) rc = pipe_plan(c, P, (const EnvDesc*)c->d_envs.ptr, (const Item*)c->d_items.ptr, nitems);
+    if (!rc) rc = pipe_plan(c, P, (const EnvDesc*)c->d_envs.ptr, (const Item*)c->d_items.ptr, plan);
     if (rc) break;
     rc = hip_check(c, hipMemsetAsync(c->d_poly_cnt.ptr, 0, 64, c->stream), "memset");
     if (rc) break;

@@ -32,6 +32,7 @@ struct hcs_batch {
   /* split triangle / soft items whose last measured candidate count exceeds
    * this into sub-items over LBVH subtrees (0: never) */
   int split_target = 2048;
+  int queues = 1; /* candidate-kernel work queues: 1, or one per XCD */
   bool has_state = false;
   DBuf envs, state, items;
   Pipe pipe;
@@ -67,6 +68,7 @@ static void batch_enqueue(hcs_batch* b, const QueryArgs& a) {
 static int batch_upload(hcs_batch* b) {
   Ctx* c = b->c;
   const size_t nitems = std::max<size_t>(1, b->plan.items.size());
+  b->pipe.nqueues = b->queues;
   int rc;
   if ((rc = ensure(c, b->envs, sizeof(EnvDesc) * b->npairs)) ||
       (rc = ensure(c, b->state, sizeof(double) * HCS_STATE * b->npairs)) ||
@@ -82,7 +84,7 @@ static int batch_upload(hcs_batch* b) {
                                      hipMemcpyHostToDevice, c->stream),
                    "upload items");
   if (!rc)
-    rc = pipe_plan(c, b->pipe, (const EnvDesc*)b->envs.ptr, (const Item*)b->items.ptr, (int)b->plan.items.size());
+    rc = pipe_plan(c, b->pipe, (const EnvDesc*)b->envs.ptr, (const Item*)b->items.ptr, b->plan);
   if (rc) return rc;
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
@@ -279,14 +281,15 @@ int hcs_batch_counts(hcs_batch* b, int64_t* counts) {
 
 int hcs_batch_config(hcs_batch* b, int32_t* cfg) {
   if (!b || !cfg) return HCS_ERR_ARG;
-  cfg[0] = b->blocks_per_cu, cfg[1] = b->qpl, cfg[2] = b->tet_max, cfg[3] = b->split_target;
+  cfg[0] = b->blocks_per_cu, cfg[1] = b->qpl, cfg[2] = b->tet_max, cfg[3] = b->split_target, cfg[4] = b->queues;
   return HCS_OK;
 }
 
 int hcs_batch_set_config(hcs_batch* b, const int32_t* cfg) {
-  if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0 || cfg[3] < 0)
+  if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0 || cfg[3] < 0 ||
+      cfg[4] < 1 || cfg[4] > HCS_QUEUES)
     return HCS_ERR_ARG;
-  b->blocks_per_cu = cfg[0], b->qpl = cfg[1], b->tet_max = cfg[2], b->split_target = cfg[3];
+  b->blocks_per_cu = cfg[0], b->qpl = cfg[1], b->tet_max = cfg[2], b->split_target = cfg[3], b->queues = cfg[4];
   int rc = batch_plan(b);
   if (!rc) rc = batch_capture(b);
   return rc;
@@ -316,13 +319,13 @@ static int time_steps(hcs_batch* b, int reps, double* ms) {
 int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
   if (!b || reps <= 0) return HCS_ERR_ARG;
   const int bpc[] = {1, 2, 4, 8};
-  const int qpls[] = {1};
-  const int tets[] = {64, 256, 1024};
+  const int queues[] = {1, HCS_QUEUES};
+  const int tets[] = {0, 64, 256};
   double best = 1e300;
-  int bb = b->blocks_per_cu, bq = b->qpl, bt = b->tet_max;
+  int bb = b->blocks_per_cu, bq = b->queues, bt = b->tet_max;
   for (int t : tets)
-    for (int q : qpls) {
-      b->tet_max = t, b->qpl = q;
+    for (int q : queues) {
+      b->tet_max = t, b->queues = q;
       int rc = batch_plan(b);
       if (rc) return rc;
       for (int k : bpc) {
@@ -333,7 +336,7 @@ int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
         if (ms < best) best = ms, bb = k, bq = q, bt = t;
       }
     }
-  b->blocks_per_cu = bb, b->qpl = bq, b->tet_max = bt;
+  b->blocks_per_cu = bb, b->queues = bq, b->tet_max = bt;
   int rc = batch_plan(b);
   if (!rc) rc = batch_capture(b);
   if (best_ms) *best_ms = best;

@@ -279,6 +279,45 @@ __global__ void k_prep(const EnvDesc* __restrict__ envs, const double* __restric
   for (int i = 0; i < 12; ++i) xf[12 * (size_t)e + i] = X[i];
 }
 
+/* AABBs of a soft mesh's tets in Morton (leaf) order. */
+__global__ void k_gather_boxes(const double* __restrict__ box, const int* __restrict__ idx, int n,
+                               double* __restrict__ boxL) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) boxL[6 * (size_t)i + c] = box[6 * (size_t)idx[i] + c];
+}
+
+/* Per step, one block per tet-query pair: its rigid triangles moved into
+ * A's frame (vertices by xform, unit normal by rot, and their AABB), the
+ * same arithmetic the candidate and clip stages use. */
+__global__ __launch_bounds__(64) void k_xtri(const EnvDesc* __restrict__ envs, const int* __restrict__ xenv,
+                                             const double* __restrict__ xf, double* __restrict__ xtri) {
+  const int e = xenv[blockIdx.x];
+  const EnvDesc& E = envs[e];
+  const double* X = xf + 12 * (size_t)e;
+  for (int t = threadIdx.x; t < E.nB; t += blockDim.x) {
+    double* o = xtri + HCS_XTRI * (size_t)(E.xtri_off + t);
+    const int32_t* f = E.FB + 3 * (size_t)t;
+    double lo[3], hi[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double y[3];
+      xform(X, E.VB + 3 * (size_t)f[i], y);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        o[3 * i + c] = y[c];
+        lo[c] = i == 0 ? y[c] : fmin(lo[c], y[c]);
+        hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
+      }
+    }
+    double nS[3];
+    rot(X, E.nrmB + 3 * (size_t)t, nS);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[9 + c] = nS[c], o[12 + c] = lo[c], o[15 + c] = hi[c];
+  }
+}
+
 /* Static boxes of one item per wave: [0..5] its query elements in their own
  * frame (tets of A in A's frame; triangles or tets of B in B's frame), then
  * [6..11] the target (B's vertex AABB in B's frame for tet queries, the
@@ -334,27 +373,43 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
 }
 
 /* Per step, one thread per item: keep the items whose static boxes may
- * overlap under this step's pose (wave-aggregated append to the live list);
- * a rejected item gets zero counts here. */
+ * overlap under this step's pose, appended to their queue's segment of the
+ * live list (one atomic per queue present in the wave); a rejected item gets
+ * zero counts here. */
 __global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
                                               int nitems, const double* __restrict__ ibox,
-                                              const double* __restrict__ xf, int* __restrict__ live,
+                                              const double* __restrict__ xf, const int* __restrict__ qseg,
+                                              int nq, int* __restrict__ live, int4* __restrict__ live_item,
                                               unsigned* __restrict__ live_cnt, int* __restrict__ ncand,
                                               unsigned* __restrict__ nsurv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   bool keep = false;
+  int q = -1;
+  Item it{0, 0, 0, 0};
   if (i < nitems) {
-    const Item it = items[i];
-    const double* q = ibox + 12 * (size_t)i;
+    it = items[i];
+    const double* bx = ibox + 12 * (size_t)i;
     const bool tet = envs[it.env].kind == KIND_TET;
-    keep = boxes_may_overlap(xf + 12 * (size_t)it.env, tet ? q + 6 : q, tet ? q : q + 6);
+    keep = boxes_may_overlap(xf + 12 * (size_t)it.env, tet ? bx + 6 : bx, tet ? bx : bx + 6);
     if (!keep) ncand[i] = 0, nsurv[i] = 0;
+    q = it.env % nq;
   }
-  const unsigned long long m = __ballot(keep);
-  unsigned base = 0;
-  if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(live_cnt, (unsigned)__popcll(m));
-  base = __shfl(base, 0, 64);
-  if (keep) live[base + lane_prefix(m)] = i;
+  unsigned long long rem = __ballot(keep);
+  while (rem) {
+    const int lead = __builtin_ctzll(rem);
+    const int qq = __shfl(q, lead, 64);
+    const unsigned long long m = __ballot(keep && q == qq);
+    unsigned base = 0;
+    if (lane == lead) base = atomicAdd(live_cnt + qq, (unsigned)__popcll(m));
+    base = __shfl(base, lead, 64);
+    if (keep && q == qq) {
+      const int pos = qseg[qq] + base + lane_prefix(m);
+      live[pos] = i;
+      live_item[pos] = make_int4(it.env, it.first, it.count, it.root);
+    }
+    rem &= ~m;
+  }
 }
 
 /* Sub-roots for splitting (one thread per request): the frontier of the LBVH
@@ -395,17 +450,64 @@ __global__ void k_split_roots(const EnvDesc* __restrict__ envs, const Item* __re
 template <bool KEYS>
 __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
-  const unsigned n = A.live ? *A.live_cnt : (unsigned)A.nitems;
   unsigned long long s_next = 0, s_end = 0; /* this wave's survivor slots */
+  const int nq = A.live ? A.nqueues : 1;
+  const int lane = threadIdx.x & 63;
   if (A.counter == nullptr) {
-    if (blockIdx.x < n) run_cand<KEYS>(L, A, A.live ? A.live[blockIdx.x] : (int)blockIdx.x, s_next, s_end);
+    /* one item per block, queues back to back */
+    unsigned v = blockIdx.x;
+    for (int x = 0; x < nq; ++x) {
+      const unsigned n = A.live ? A.live_cnt[x] : (unsigned)A.nitems;
+      if (v < n) {
+        if (A.live) {
+          const int4 r = A.live_item[A.qseg[x] + v];
+          run_cand<KEYS>(L, A, A.live[A.qseg[x] + v], Item{r.x, r.y, r.z, r.w}, s_next, s_end);
+        } else {
+          run_cand<KEYS>(L, A, (int)v, A.items[v], s_next, s_end);
+        }
+        break;
+      }
+      v -= n;
+    }
+  } else if (A.live) {
+    /* persistent: the block's own queue first, then the others.  The next
+     * item's record is loaded, and the work counter advanced for the one
+     * after, while the current item runs, so neither round trip is on an
+     * item's critical path. */
+    const int x0 = (int)(blockIdx.x % nq);
+    for (int k = 0; k < nq; ++k) {
+      const int x = (x0 + k) % nq;
+      const unsigned n = A.live_cnt[x];
+      const int base = A.qseg[x];
+      unsigned raw = 0;
+      if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
+      unsigned v = __shfl(raw, 0, 64);
+      if (v >= n) continue;
+      int idx = A.live[base + v];
+      int4 rec = A.live_item[base + v];
+      if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
+      for (;;) {
+        const unsigned vn = __shfl(raw, 0, 64);
+        int nidx = 0;
+        int4 nrec = make_int4(0, 0, 0, 0);
+        if (vn < n) {
+          nidx = A.live[base + vn];
+          nrec = A.live_item[base + vn];
+          if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
+        }
+        run_cand<KEYS>(L, A, idx, Item{rec.x, rec.y, rec.z, rec.w}, s_next, s_end);
+        if (vn >= n) break;
+        idx = nidx;
+        rec = nrec;
+      }
+    }
   } else {
     for (;;) {
       unsigned v = 0;
-      if (threadIdx.x == 0) v = atomicAdd(A.counter, 1u);
+      if (lane == 0) v = atomicAdd(A.counter, 1u);
       v = __shfl(v, 0, 64);
-      if (v >= n) break;
-      run_cand<KEYS>(L, A, A.live ? A.live[v] : (int)v, s_next, s_end);
+      if (v >= (unsigned)A.nitems) break;
+      run_cand<KEYS>(L, A, (int)v, A.items[v], s_next, s_end);
     }
   }
   fill_sentinels(A, s_next, s_end);
@@ -511,21 +613,29 @@ QueryArgs pipe_args(const Pipe& p, const EnvDesc* envs, const double* state, con
   a.item_ncand = (int*)p.ncand.ptr;
   a.item_nsurv = (unsigned*)p.nsurv.ptr;
   a.live = p.live.ptr ? (const int*)p.live.ptr : nullptr;
-  a.live_cnt = p.live.ptr ? (const unsigned*)p.counter.ptr + 1 : nullptr;
+  a.live_cnt = p.live.ptr ? (const unsigned*)p.counter.ptr + HCS_QUEUES : nullptr;
+  a.qseg = p.live.ptr ? (const int*)p.qseg.ptr : nullptr;
+  a.nqueues = p.nqueues;
+  a.live_item = p.live.ptr ? (const int4*)p.live_item.ptr : nullptr;
+  a.xtri = (const double*)p.xtri.ptr;
   return a;
 }
 
 void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs) {
   hipStream_t s = c->stream;
-  launch_fill_u32(s, (unsigned*)p.counter.ptr, 0u, 2);
+  launch_fill_u32(s, (unsigned*)p.counter.ptr, 0u, 2 * HCS_QUEUES);
   launch_fill_u32(s, (unsigned*)p.surv_cnt.ptr, 0u, 2);
   launch_fill_u32(s, (unsigned*)p.err.ptr, 0u, 1);
   launch_fill_u32(s, (unsigned*)p.nsurv.ptr + a.nitems, 0u, 1);
   launch_prep(s, a.envs, a.state, (double*)p.xf.ptr, npairs);
+  if (p.nxenv > 0)
+    hipLaunchKernelGGL(k_xtri, dim3(p.nxenv), dim3(64), 0, s, a.envs, (const int*)p.xenv.ptr,
+                       (const double*)p.xf.ptr, (double*)p.xtri.ptr);
   if (a.nitems > 0 && a.live)
     hipLaunchKernelGGL(k_cull, dim3((a.nitems + 255) / 256), dim3(256), 0, s, a.envs, a.items, a.nitems,
-                       (const double*)p.ibox.ptr, (const double*)p.xf.ptr, (int*)p.live.ptr,
-                       (unsigned*)p.counter.ptr + 1, a.item_ncand, a.item_nsurv);
+                       (const double*)p.ibox.ptr, (const double*)p.xf.ptr, (const int*)p.qseg.ptr, p.nqueues,
+                       (int*)p.live.ptr, (int4*)p.live_item.ptr, (unsigned*)p.counter.ptr + HCS_QUEUES,
+                       a.item_ncand, a.item_nsurv);
 }
 
 void launch_cand(hipStream_t s, const QueryArgs& a, int grid) {
@@ -574,18 +684,40 @@ void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cli
   pipe_back(c, p, a, npairs, clip_grid, 0, nullptr, nullptr, 0, e_clip, e_reduce);
 }
 
-int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, int nitems) {
+int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Plan& plan) {
   int rc;
+  const std::vector<Item>& host_items = plan.items;
+  const int nitems = (int)host_items.size();
   const size_t ni = (size_t)std::max(1, nitems);
-  if ((rc = ensure(c, p.ibox, sizeof(double) * 12 * ni)) || (rc = ensure(c, p.live, sizeof(int) * ni))) return rc;
+  if ((rc = ensure(c, p.ibox, sizeof(double) * 12 * ni)) || (rc = ensure(c, p.live, sizeof(int) * ni)) ||
+      (rc = ensure(c, p.live_item, sizeof(Item) * ni)) || (rc = ensure(c, p.qseg, sizeof(int) * (HCS_QUEUES + 1))) ||
+      (rc = ensure(c, p.xtri, sizeof(double) * HCS_XTRI * std::max<int64_t>(1, plan.ntx))) ||
+      (rc = ensure(c, p.xenv, sizeof(int) * std::max<size_t>(1, plan.xenv.size()))))
+    return rc;
+  /* queue segments of the live list: queue x holds the items of pairs
+   * e % nqueues == x */
+  p.nqueues = std::max(1, std::min(p.nqueues, HCS_QUEUES));
+  std::vector<int> seg(HCS_QUEUES + 1, 0);
+  for (const Item& it : host_items) ++seg[it.env % p.nqueues + 1];
+  for (int x = 0; x < HCS_QUEUES; ++x) seg[x + 1] += seg[x];
+  p.qseg_host = seg;
+  p.nxenv = (int)plan.xenv.size();
+  rc = hip_check(c, hipMemcpyAsync(p.qseg.ptr, p.qseg_host.data(), sizeof(int) * (HCS_QUEUES + 1),
+                                   hipMemcpyHostToDevice, c->stream), "upload queues");
+  if (!rc && p.nxenv > 0)
+    rc = hip_check(c, hipMemcpyAsync(p.xenv.ptr, plan.xenv.data(), sizeof(int) * p.nxenv, hipMemcpyHostToDevice,
+                                     c->stream), "upload tet-query pairs");
+  if (rc) return rc;
   if (nitems > 0) hipLaunchKernelGGL(k_item_boxes, dim3(nitems), dim3(64), 0, c->stream, envs, items, nitems,
                                      (double*)p.ibox.ptr);
-  return hip_check(c, hipGetLastError(), "item boxes");
+  rc = hip_check(c, hipGetLastError(), "item boxes");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  return rc;
 }
 
 void pipe_free(Pipe& p) {
   for (DBuf* b : {&p.counter, &p.surv, &p.surv_cnt, &p.ncand, &p.nsurv, &p.off, &p.scan_tmp, &p.contrib, &p.out, &p.err,
-                  &p.xf, &p.ibox, &p.live})
+                  &p.xf, &p.ibox, &p.live, &p.live_item, &p.qseg, &p.xtri, &p.xenv})
     if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
 }
 
@@ -639,6 +771,9 @@ static int build_bvh(Ctx* c, Mesh& m) {
     HCHK(c, hipMalloc((void**)&m.nodes, sizeof(Node)));
     const int32_t zero = 0;
     int rc = upload(c, (void**)&m.leaf, &zero, sizeof(int32_t));
+    if (!rc) rc = hip_check(c, hipMalloc((void**)&m.boxL, 6 * sizeof(double)), "hipMalloc");
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(m.boxL, m.box, 6 * sizeof(double), hipMemcpyDeviceToDevice, c->stream), "copy");
     return rc ? rc : hip_check(c, hipStreamSynchronize(c->stream), "sync");
   }
   /* Scene bounds from the vertices (they bound every tet AABB). */
@@ -691,6 +826,14 @@ static int build_bvh(Ctx* c, Mesh& m) {
     m.height = h;
   }
   if (rc == HCS_OK) {
+    fail(hipMalloc((void**)&m.boxL, 6 * sizeof(double) * (size_t)n), "hipMalloc");
+    if (rc == HCS_OK) {
+      hipLaunchKernelGGL(k_gather_boxes, dim3((n + 255) / 256), dim3(256), 0, c->stream, m.box, idx2, n, m.boxL);
+      fail(hipGetLastError(), "k_gather_boxes");
+      fail(hipStreamSynchronize(c->stream), "sync");
+    }
+  }
+  if (rc == HCS_OK) {
     m.leaf = idx2; /* kept: Morton order of the tets */
     idx2 = nullptr;
   }
@@ -704,7 +847,7 @@ static int build_bvh(Ctx* c, Mesh& m) {
 
 static void free_mesh(Mesh& m) {
   for (void* p : {(void*)m.V, (void*)m.E, (void*)m.p, (void*)m.rec, (void*)m.box, (void*)m.nodes, (void*)m.nrm,
-                  (void*)m.leaf})
+                  (void*)m.leaf, (void*)m.boxL})
     if (p) (void)hipFree(p);
   m = Mesh();
 }
@@ -712,6 +855,8 @@ static void free_mesh(Mesh& m) {
 int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int qpl, int tet_max, Plan& plan) {
   plan.envs.assign(npairs, EnvDesc{});
   plan.items.clear();
+  plan.xenv.clear();
+  plan.ntx = 0;
   plan.pair_kind.assign(npairs, 0);
   const int per_item = 64 * std::max(1, qpl);
   for (int e = 0; e < npairs; ++e) {
@@ -725,7 +870,9 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
       return set_err(c, HCS_ERR_ARG,
                      "pair " + std::to_string(e) + ": geom_a must be soft (pass soft-rigid pairs soft first)");
     EnvDesc& E = plan.envs[e];
-    E.recA = A.rec, E.boxA = A.box, E.nodesA = A.nodes, E.leafA = A.leaf, E.TA = A.E, E.VA = A.V, E.ntA = A.nelem;
+    E.recA = A.rec, E.boxA = A.box, E.nodesA = A.nodes, E.leafA = A.leaf, E.boxLA = A.boxL, E.TA = A.E, E.VA = A.V,
+    E.ntA = A.nelem;
+    E.xtri_off = -1;
     E.VB = B.V, E.FB = B.E, E.nrmB = B.nrm, E.recB = B.rec, E.nB = B.nelem, E.nvB = B.nv;
     E.b_soft = B.kind == 1;
     E.a_is_M = A.geom_id < B.geom_id;
@@ -763,6 +910,9 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
       E.kind = KIND_TET;
       nq = A.nelem;
       chunk = 64;
+      E.xtri_off = plan.ntx;
+      plan.ntx += B.nelem;
+      plan.xenv.push_back(e);
     } else {
       E.kind = KIND_TRI;
       nq = B.nelem;
@@ -975,7 +1125,7 @@ static int run_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_o
   const int nitems = (int)plan.items.size();
   Pipe& P = c->pipe;
   if ((rc = pipe_reserve(c, P, nitems, npairs, std::max<int64_t>(P.cap, 4096), 0))) return rc;
-  if ((rc = pipe_plan(c, P, (const EnvDesc*)c->d_envs.ptr, (const Item*)c->d_items.ptr, nitems))) return rc;
+  if ((rc = pipe_plan(c, P, (const EnvDesc*)c->d_envs.ptr, (const Item*)c->d_items.ptr, plan))) return rc;
   if ((rc = ensure(c, c->d_face_cnt, 64)) || (rc = ensure(c, c->d_poly_cnt, 64)) || (rc = ensure(c, c->d_cand_cnt, 64)))
     return rc;
   const bool keep = opts && opts->keep_polygons && !keys;

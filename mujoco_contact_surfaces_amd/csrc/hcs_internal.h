@@ -25,6 +25,7 @@ struct Mesh {
   double* box = nullptr;
   Node* nodes = nullptr;
   int32_t* leaf = nullptr; /* tets in Morton (BVH leaf) order */
+  double* boxL = nullptr;   /* their AABBs in that order */
   double* nrm = nullptr;
 };
 
@@ -37,10 +38,13 @@ struct DBuf {
 /* The per-step pipeline (hcs_engine.hip): k_prep, k_cand, scan of
  * per-item survivor counts, k_clip, k_env_reduce. */
 struct Pipe {
-  DBuf counter /* [0] work queue, [1] live items */, surv, surv_cnt, ncand, nsurv, off, scan_tmp, contrib, out, err,
-      xf, ibox, live;
+  DBuf counter /* [0..7] work counters, [8..15] live items per queue */, surv, surv_cnt, ncand, nsurv, off, scan_tmp,
+      contrib, out, err, xf, ibox, live, live_item, qseg, xtri, xenv;
   int64_t cap = 0;
   size_t scan_bytes = 0;
+  std::vector<int> qseg_host; /* host copy of the queue segments (the upload source) */
+  int nqueues = 1;            /* candidate-kernel work queues (1..HCS_QUEUES) */
+  int nxenv = 0;              /* pairs in tet-query mode (their triangles are moved per step) */
 };
 
 struct Ctx;
@@ -53,6 +57,8 @@ struct Plan {
   std::vector<EnvDesc> envs;
   std::vector<Item> items;
   std::vector<int> pair_kind;
+  std::vector<int> xenv; /* pairs in tet-query mode */
+  int64_t ntx = 0;       /* their triangles in total */
 };
 
 struct Ctx {
@@ -92,7 +98,8 @@ void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid
                   const double* dstar_dE, const double* invE, bool cand_only, int dir0);
 /* Static boxes of the current items (once per plan, after the items and
  * env descriptors are on the device). */
-int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, int nitems);
+struct Plan;
+int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Plan& plan);
 /* Counter resets, pose composition and the item cull of one step. */
 void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs);
 /* For each requested item (index into the device items), the LBVH

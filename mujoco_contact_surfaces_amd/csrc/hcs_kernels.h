@@ -8,7 +8,8 @@
  *                               constant c with p(x) = c + g.x, unit g, pad)
  *               box   nt x 6   AABB lo, hi (soft frame)
  *               nodes (nt-1) x Node   binary LBVH, child boxes stored in the
- *                               parent so one 112-byte fetch tests both
+ *                               parent so one 128-byte fetch (one cache
+ *                               line) tests both
  *               leaf  nt int32 tets in Morton order
  *               T     nt x 4 int32, V nv x 3, p nv
  *   rigid mesh: V nv x 3, F ntri x 3 int32, nrm ntri x 3 (unit normals)
@@ -30,12 +31,21 @@ namespace hcs {
 #define HCS_FREC 12
 #define HCS_PREC 40
 #define HCS_STATE 36
+/* a rigid triangle moved into A's frame: 3 vertices, unit normal, AABB */
+#define HCS_XTRI 18
+/* at most this many work queues for the candidate kernel: with one per XCD
+ * the waves of one XCD (workgroups are dispatched to the 8 XCDs round-robin
+ * by block index) work on the same pairs and share that XCD's L2 */
+#define HCS_QUEUES 8
 
+/* 128 bytes, so a node on a 128-byte boundary is one cache line. */
 struct Node {
-  double b[12];  /* child 0 lo[3] hi[3], child 1 lo[3] hi[3] */
-  int32_t c[2];  /* >= 0 internal node, < 0 leaf: ~element */
-  int32_t h[2];  /* subtree heights (leaf 0) */
+  double b[12];   /* child 0 lo[3] hi[3], child 1 lo[3] hi[3] */
+  int32_t c[2];   /* >= 0 internal node, < 0 leaf: ~element */
+  int32_t h[2];   /* subtree heights (leaf 0) */
+  int32_t pad[4];
 };
+static_assert(sizeof(Node) == 128, "one LBVH node per cache line");
 
 enum EnvKind { KIND_TRI = 0, KIND_TET = 1, KIND_SOFT = 2 };
 
@@ -44,6 +54,7 @@ struct EnvDesc {
   const double* boxA;
   const Node* nodesA;
   const int32_t* leafA; /* A's tets in Morton order (tet-query items) */
+  const double* boxLA;  /* their AABBs in that order */
   const int32_t* TA;
   const double* VA;
   const double* VB;
@@ -54,7 +65,8 @@ struct EnvDesc {
   int32_t a_is_M, repr, apply, b_soft;
   double mu, dstar, dt;
   int32_t item_first, item_count;
-  int32_t pad[2];
+  int64_t xtri_off; /* tet-query mode: first of this pair's triangles in the
+                       per-step moved-triangle buffer (HCS_XTRI doubles each) */
 };
 
 /* A work item: queries first .. first+count-1 of pair env.  Triangle and
@@ -77,7 +89,7 @@ struct QueryArgs {
   const Item* items;
   int32_t nitems;
   int32_t queries_per_lane;
-  unsigned int* counter; /* persistent work queue; nullptr: item = blockIdx.x */
+  unsigned int* counter; /* per-queue work counters (persistent kernel); nullptr: one item per block */
   double* face_rec;
   int64_t* face_key;
   unsigned long long* face_cnt;
@@ -100,9 +112,15 @@ struct QueryArgs {
   int64_t surv_cap;
   int* item_ncand;
   unsigned* item_nsurv;
-  /* items whose static boxes overlap this step (nullptr: all items) */
+  /* items whose static boxes overlap this step, in nqueues queues (the
+   * items of pair e go to queue e % nqueues; queue x holds live_cnt[x]
+   * items from live + qseg[x]); nullptr: all items, one queue */
   const int* live;
   const unsigned* live_cnt;
+  const int* qseg;
+  int32_t nqueues;
+  const int4* live_item; /* the live items' Item records, same order as live */
+  const double* xtri;    /* per-step moved triangles of tet-query pairs */
 };
 
 /* Inputs of the clip kernel beyond QueryArgs. */

@@ -551,12 +551,11 @@ __device__ inline void fill_sentinels(const QueryArgs& A, unsigned long long fro
  * land in.  KEYS: write the canonical candidate keys instead (no tests, no
  * survivors). */
 template <bool KEYS>
-__device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, unsigned long long& s_next,
-                                unsigned long long& s_end) {
+__device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const Item item,
+                                unsigned long long& s_next, unsigned long long& s_end) {
   const int lane = threadIdx.x & 63;
   const unsigned long long t_begin = A.item_stats ? clock64() : 0ull;
   unsigned long long steps = 0;
-  const Item item = A.items[it];
   const int env = item.env;
   const EnvDesc E = A.envs[env];
   __syncthreads();
@@ -584,10 +583,11 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, unsigned
      * union box is compact and culls most rigid triangles */
     valid = lane < item.count;
     b = valid ? E.leafA[item.first + lane] : 0;
+    const double* lb = E.boxLA + 6 * (size_t)(valid ? item.first + lane : 0); /* = boxA of tet b */
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      bx[c] = valid ? E.boxA[6 * (size_t)b + c] : __builtin_inf();
-      bx[3 + c] = valid ? E.boxA[6 * (size_t)b + 3 + c] : -__builtin_inf();
+      bx[c] = valid ? lb[c] : __builtin_inf();
+      bx[3 + c] = valid ? lb[3 + c] : -__builtin_inf();
       ulo[c] = wave_min(bx[c]);
       uhi[c] = wave_max(bx[3 + c]);
     }
@@ -650,23 +650,32 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, unsigned
             bool touch = false;
             __syncthreads();
             if (lane < ntile) {
-              const int32_t* f = E.FB + 3 * (size_t)(t0 + lane);
               double l3[3], h3[3];
+              if (A.xtri && E.xtri_off >= 0) {
+                /* moved once per step for the whole pair (k_xtri) */
+                const double* o = A.xtri + HCS_XTRI * (size_t)(E.xtri_off + t0 + lane);
 #pragma unroll
-              for (int i = 0; i < 3; ++i) {
-                double y[3];
-                xform(X, E.VB + 3 * (size_t)f[i], y);
+                for (int k = 0; k < 9; ++k) L.tv[9 * lane + k] = o[k];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                  L.tv[9 * lane + 3 * i + c] = y[c];
-                  l3[c] = i == 0 ? y[c] : fmin(l3[c], y[c]);
-                  h3[c] = i == 0 ? y[c] : fmax(h3[c], y[c]);
+                for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = o[9 + c], l3[c] = o[12 + c], h3[c] = o[15 + c];
+              } else {
+                const int32_t* f = E.FB + 3 * (size_t)(t0 + lane);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                  double y[3];
+                  xform(X, E.VB + 3 * (size_t)f[i], y);
+#pragma unroll
+                  for (int c = 0; c < 3; ++c) {
+                    L.tv[9 * lane + 3 * i + c] = y[c];
+                    l3[c] = i == 0 ? y[c] : fmin(l3[c], y[c]);
+                    h3[c] = i == 0 ? y[c] : fmax(h3[c], y[c]);
+                  }
                 }
-              }
-              double nS[3];
-              rot(X, E.nrmB + 3 * (size_t)(t0 + lane), nS);
+                double nS[3];
+                rot(X, E.nrmB + 3 * (size_t)(t0 + lane), nS);
 #pragma unroll
-              for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
+                for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
+              }
 #pragma unroll
               for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = l3[c], L.box[6 * lane + 3 + c] = h3[c];
               const double ub[6] = {ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2]};
