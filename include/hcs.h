@@ -186,10 +186,12 @@ int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
  * reps: ms[0] candidate kernel (traversal, cheap tests, compaction),
  * ms[1] clip/force kernel, ms[2] scan + per-pair reduction. */
 int hcs_batch_kernel_times(hcs_batch* batch, int32_t reps, double* ms);
-/* Per-item profile of one eager step (shader clock): out (capacity x 8) =
- * [pair, kind (0 triangle queries, 1 tet queries, 2 soft-soft), queries,
- * cycles, production steps, candidates, cheap-test survivors, n elements of
- * B]; *nitems = number of items (capacity 0 queries it). */
+/* Per-item profile of one eager step: out (capacity x 10) = [pair, kind
+ * (0 triangle queries, 1 tet queries, 2 soft-soft), queries, shader cycles,
+ * production steps, candidates, cheap-test survivors, n elements of B,
+ * wall-clock start, wall-clock end (constant-rate device counter; 0 for
+ * items culled in that step)]; *nitems = number of items (capacity 0
+ * queries it). */
 int hcs_batch_item_stats(hcs_batch* batch, int64_t* out, int64_t capacity, int64_t* nitems);
 /* Work-unit counts of the last step: [candidates, polygons, faces]. */
 int hcs_batch_counts(hcs_batch* batch, int64_t* counts);

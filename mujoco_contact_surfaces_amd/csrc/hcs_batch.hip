@@ -95,19 +95,42 @@ static int batch_upload(hcs_batch* b) {
  * new plan (at most 4 rounds) while some item still exceeds it.  The
  * frontier's subtrees hold each leaf exactly once, so the candidate set is
  * unchanged. */
+/* One eager step of the current plan; per-item candidate counts into nc. */
+static int batch_measure(hcs_batch* b, std::vector<int>& nc) {
+  Ctx* c = b->c;
+  const int n = (int)b->plan.items.size();
+  nc.assign(n, 0);
+  if (n == 0) return HCS_OK;
+  batch_enqueue(b, batch_args(b));
+  int rc = hip_check(c, hipGetLastError(), "measure step");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(nc.data(), b->pipe.ncand.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream),
+                   "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  return rc;
+}
+
+/* Work order of the items for the candidate kernel: decreasing measured
+ * candidate count (longest first), ties by item index. */
+static int batch_order(hcs_batch* b) {
+  std::vector<int> nc;
+  int rc = batch_measure(b, nc);
+  if (rc) return rc;
+  std::vector<int>& o = b->plan.order;
+  o.resize(nc.size());
+  for (size_t i = 0; i < o.size(); ++i) o[i] = (int)i;
+  std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return nc[x] > nc[y]; });
+  return batch_upload(b);
+}
+
 static int batch_split(hcs_batch* b) {
   Ctx* c = b->c;
   if (b->split_target <= 0) return HCS_OK;
   for (int round = 0; round < 4; ++round) {
     const int n = (int)b->plan.items.size();
     if (n == 0) return HCS_OK;
-    batch_enqueue(b, batch_args(b));
-    std::vector<int> nc(n);
-    int rc = hip_check(c, hipGetLastError(), "measure step");
-    if (!rc)
-      rc = hip_check(c, hipMemcpyAsync(nc.data(), b->pipe.ncand.ptr, sizeof(int) * n, hipMemcpyDeviceToHost,
-                                       c->stream), "dl");
-    if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+    std::vector<int> nc;
+    int rc = batch_measure(b, nc);
     if (rc) return rc;
     std::vector<int> req, depth, slot(n, -1);
     for (int i = 0; i < n; ++i) {
@@ -147,12 +170,14 @@ static int batch_split(hcs_batch* b) {
 }
 
 /* (Re)builds the plan for the current shape and uploads it; with a state
- * set, heavy items are split from one measured step. */
+ * set, heavy items are split and all items ordered by measured steps. */
 static int batch_plan(hcs_batch* b) {
   batch_free_graph(b);
   int rc = build_plan(b->c, b->npairs, b->geoms.data(), &b->opts, b->qpl, b->tet_max, b->plan);
+  b->plan.order.clear();
   if (!rc) rc = batch_upload(b);
   if (!rc && b->has_state) rc = batch_split(b);
+  if (!rc && b->has_state) rc = batch_order(b);
   return rc;
 }
 
@@ -318,7 +343,7 @@ static int time_steps(hcs_batch* b, int reps, double* ms) {
 
 int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
   if (!b || reps <= 0) return HCS_ERR_ARG;
-  const int bpc[] = {1, 2, 4, 8};
+  const int bpc[] = {2, 4, 8, 12};
   const int queues[] = {1, HCS_QUEUES};
   const int tets[] = {0, 64, 256};
   double best = 1e300;
@@ -350,15 +375,15 @@ int hcs_batch_item_stats(hcs_batch* b, int64_t* out, int64_t capacity, int64_t* 
   *nitems = n;
   if (capacity < n) return capacity == 0 ? HCS_OK : set_err(c, HCS_ERR_CAPACITY, "item stats buffer too small");
   DBuf d;
-  int rc = ensure(c, d, sizeof(unsigned long long) * 4 * std::max(1, n));
+  int rc = ensure(c, d, sizeof(unsigned long long) * 6 * std::max(1, n));
   if (rc) return rc;
-  rc = hip_check(c, hipMemsetAsync(d.ptr, 0, sizeof(unsigned long long) * 4 * std::max(1, n), c->stream),
+  rc = hip_check(c, hipMemsetAsync(d.ptr, 0, sizeof(unsigned long long) * 6 * std::max(1, n), c->stream),
                  "memset");
   if (rc) return rc;
   QueryArgs a = batch_args(b);
   a.item_stats = (unsigned long long*)d.ptr;
   batch_enqueue(b, a);
-  std::vector<unsigned long long> h(4 * (size_t)n);
+  std::vector<unsigned long long> h(6 * (size_t)n);
   rc = hip_check(c, hipGetLastError(), "stats step");
   if (!rc) rc = hip_check(c, hipMemcpyAsync(h.data(), d.ptr, h.size() * 8, hipMemcpyDeviceToHost, c->stream), "dl");
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
@@ -366,10 +391,12 @@ int hcs_batch_item_stats(hcs_batch* b, int64_t* out, int64_t capacity, int64_t* 
   if (rc) return rc;
   for (int i = 0; i < n; ++i) {
     const Item& it = b->plan.items[i];
-    int64_t* o = out + 8 * (size_t)i;
+    int64_t* o = out + 10 * (size_t)i;
+    const unsigned long long* q = &h[6 * (size_t)i];
     o[0] = it.env, o[1] = b->plan.envs[it.env].kind, o[2] = it.count;
-    o[3] = (int64_t)h[4 * i], o[4] = (int64_t)h[4 * i + 1], o[5] = (int64_t)h[4 * i + 2], o[6] = (int64_t)h[4 * i + 3];
+    o[3] = (int64_t)q[0], o[4] = (int64_t)q[1], o[5] = (int64_t)q[2], o[6] = (int64_t)q[3];
     o[7] = b->plan.envs[it.env].nB;
+    o[8] = (int64_t)q[4], o[9] = (int64_t)q[5];
   }
   return HCS_OK;
 }

@@ -377,17 +377,21 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
  * live list (one atomic per queue present in the wave); a rejected item gets
  * zero counts here. */
 __global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
-                                              int nitems, const double* __restrict__ ibox,
+                                              const int* __restrict__ order, int nitems,
+                                              const double* __restrict__ ibox,
                                               const double* __restrict__ xf, const int* __restrict__ qseg,
                                               int nq, int* __restrict__ live, int4* __restrict__ live_item,
                                               unsigned* __restrict__ live_cnt, int* __restrict__ ncand,
                                               unsigned* __restrict__ nsurv) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  /* threads visit the items in the plan's work order (heaviest first), so
+   * the live queues hand out long items before short ones */
+  const int i = (k < nitems && order) ? order[k] : k;
   bool keep = false;
   int q = -1;
   Item it{0, 0, 0, 0};
-  if (i < nitems) {
+  if (k < nitems) {
     it = items[i];
     const double* bx = ibox + 12 * (size_t)i;
     const bool tet = envs[it.env].kind == KIND_TET;
@@ -632,7 +636,8 @@ void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs) {
     hipLaunchKernelGGL(k_xtri, dim3(p.nxenv), dim3(64), 0, s, a.envs, (const int*)p.xenv.ptr,
                        (const double*)p.xf.ptr, (double*)p.xtri.ptr);
   if (a.nitems > 0 && a.live)
-    hipLaunchKernelGGL(k_cull, dim3((a.nitems + 255) / 256), dim3(256), 0, s, a.envs, a.items, a.nitems,
+    hipLaunchKernelGGL(k_cull, dim3((a.nitems + 255) / 256), dim3(256), 0, s, a.envs, a.items,
+                       p.ordered ? (const int*)p.order.ptr : nullptr, a.nitems,
                        (const double*)p.ibox.ptr, (const double*)p.xf.ptr, (const int*)p.qseg.ptr, p.nqueues,
                        (int*)p.live.ptr, (int4*)p.live_item.ptr, (unsigned*)p.counter.ptr + HCS_QUEUES,
                        a.item_ncand, a.item_nsurv);
@@ -702,6 +707,13 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
   for (int x = 0; x < HCS_QUEUES; ++x) seg[x + 1] += seg[x];
   p.qseg_host = seg;
   p.nxenv = (int)plan.xenv.size();
+  p.ordered = (int)plan.order.size() == nitems && nitems > 0;
+  if (p.ordered) {
+    if ((rc = ensure(c, p.order, sizeof(int) * ni))) return rc;
+    rc = hip_check(c, hipMemcpyAsync(p.order.ptr, plan.order.data(), sizeof(int) * nitems, hipMemcpyHostToDevice,
+                                     c->stream), "upload order");
+    if (rc) return rc;
+  }
   rc = hip_check(c, hipMemcpyAsync(p.qseg.ptr, p.qseg_host.data(), sizeof(int) * (HCS_QUEUES + 1),
                                    hipMemcpyHostToDevice, c->stream), "upload queues");
   if (!rc && p.nxenv > 0)
@@ -717,7 +729,7 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
 
 void pipe_free(Pipe& p) {
   for (DBuf* b : {&p.counter, &p.surv, &p.surv_cnt, &p.ncand, &p.nsurv, &p.off, &p.scan_tmp, &p.contrib, &p.out, &p.err,
-                  &p.xf, &p.ibox, &p.live, &p.live_item, &p.qseg, &p.xtri, &p.xenv})
+                  &p.xf, &p.ibox, &p.live, &p.live_item, &p.qseg, &p.xtri, &p.xenv, &p.order})
     if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
 }
 

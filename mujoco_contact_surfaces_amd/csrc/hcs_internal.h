@@ -39,12 +39,13 @@ struct DBuf {
  * per-item survivor counts, k_clip, k_env_reduce. */
 struct Pipe {
   DBuf counter /* [0..7] work counters, [8..15] live items per queue */, surv, surv_cnt, ncand, nsurv, off, scan_tmp,
-      contrib, out, err, xf, ibox, live, live_item, qseg, xtri, xenv;
+      contrib, out, err, xf, ibox, live, live_item, qseg, xtri, xenv, order;
   int64_t cap = 0;
   size_t scan_bytes = 0;
   std::vector<int> qseg_host; /* host copy of the queue segments (the upload source) */
   int nqueues = 1;            /* candidate-kernel work queues (1..HCS_QUEUES) */
   int nxenv = 0;              /* pairs in tet-query mode (their triangles are moved per step) */
+  bool ordered = false;       /* order holds a work order of the items */
 };
 
 struct Ctx;
@@ -59,6 +60,7 @@ struct Plan {
   std::vector<int> pair_kind;
   std::vector<int> xenv; /* pairs in tet-query mode */
   int64_t ntx = 0;       /* their triangles in total */
+  std::vector<int> order; /* items by decreasing measured work (empty: item order) */
 };
 
 struct Ctx {

@@ -104,7 +104,8 @@ struct QueryArgs {
   int64_t cand_cap;
   int* error;
   /* optional per-item profile: cycles, production steps, candidates,
-   * survivors of the cheap tests (4 x uint64 per item) */
+   * survivors of the cheap tests, wall-clock start and end (6 x uint64 per
+   * item) */
   unsigned long long* item_stats;
   /* survivors of the cheap tests: (elem A, elem B, item, seq in item) */
   int4* surv;

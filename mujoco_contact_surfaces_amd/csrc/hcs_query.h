@@ -523,7 +523,17 @@ __device__ inline bool boxes_may_overlap(const double* X, const double* P, const
 /* Per-wave shared memory: candidate queue, survivor queue, rigid-triangle
  * tile (tet queries) or query boxes per slot (triangle / soft queries), the
  * work-donation mailbox and the pair transform. */
+/* Traversal stack: the top HCS_LSTACK pending nodes of every lane live in
+ * LDS (slot-major, so a wave's accesses to one slot are bank-conflict free);
+ * deeper entries spill, oldest first, to a private overflow array of
+ * HCS_STACK - HCS_LSTACK entries.  DFS over these LBVHs keeps well under 16
+ * nodes pending, so the overflow is not touched in practice. */
+#ifndef HCS_LSTACK
+#define HCS_LSTACK 16
+#endif
+
 struct CandLds {
+  int stk[HCS_LSTACK * 64];
   int2 q1[192];
   int2 q2[128];
   double box[64 * 6];
@@ -555,6 +565,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
                                 unsigned long long& s_next, unsigned long long& s_end) {
   const int lane = threadIdx.x & 63;
   const unsigned long long t_begin = A.item_stats ? clock64() : 0ull;
+  const unsigned long long w_begin = A.item_stats ? wall_clock64() : 0ull;
   unsigned long long steps = 0;
   const int env = item.env;
   const EnvDesc E = A.envs[env];
@@ -568,8 +579,8 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
   const bool tet_mode = E.kind == KIND_TET;
   bool producing = true;
   /* traversal state (triangle / soft queries) */
-  int stk[HCS_STACK];
-  int bp = 0, sp = 0, visits = 0, b = 0;
+  int ovf[HCS_STACK - HCS_LSTACK];
+  int bp = 0, sp = 0, on = 0, visits = 0, b = 0; /* LDS window [bp, sp) (circular), on spilled */
   const int visit_cap = E.ntA * 64;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   /* tile state (tet queries) */
@@ -627,8 +638,8 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
       if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(leaf, b);
       q1n += __popcll(m);
     } else if (valid) {
-      stk[0] = item.root;
-      sp = 1;
+      L.stk[lane] = item.root;
+      bp = 0, sp = 1, on = 0;
     }
   }
   for (;;) {
@@ -694,19 +705,21 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
           q1n += __popcll(m);
         }
       } else {
-        int cnt = sp - bp;
+        int cnt = (sp - bp) + on;
         if (!__any(cnt > 0)) {
           producing = false;
         } else {
           /* work stealing: the k-th idle lane takes the bottom (shallowest,
-           * largest) pending subtree of the k-th lane holding two or more */
-          const unsigned long long idle = __ballot(cnt == 0), donors = __ballot(cnt >= 2);
+           * largest) pending subtree in LDS of the k-th lane holding two or
+           * more */
+          const bool donor = cnt >= 2 && sp > bp;
+          const unsigned long long idle = __ballot(cnt == 0), donors = __ballot(donor);
           const int npair = min(__popcll(idle), __popcll(donors));
           if (npair > 0) {
-            if (cnt >= 2) {
+            if (donor) {
               const int r = (int)lane_prefix(donors);
               if (r < npair) {
-                L.xfer[r] = make_int2(b - item.first, stk[bp % HCS_STACK]);
+                L.xfer[r] = make_int2(b - item.first, L.stk[(bp % HCS_LSTACK) * 64 + lane]);
                 ++bp;
                 --cnt;
               }
@@ -719,9 +732,8 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
                 b = item.first + w.x;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) lo[c] = L.box[6 * w.x + c], hi[c] = L.box[6 * w.x + 3 + c];
-                bp = sp = 0;
-                stk[0] = w.y;
-                sp = 1;
+                L.stk[lane] = w.y;
+                bp = 0, sp = 1, on = 0;
                 cnt = 1;
               }
             }
@@ -732,26 +744,37 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
           int c0 = -1, c1 = -1;
           if (cnt > 0 && ++visits > visit_cap) {
             atomicOr(A.error, 2);
-            bp = sp = 0;
+            bp = sp = on = 0;
             cnt = 0;
           }
           if (cnt > 0) {
-            const int nd = stk[(--sp) % HCS_STACK];
+            const int nd = sp > bp ? L.stk[((--sp) % HCS_LSTACK) * 64 + lane] : ovf[--on];
             const Node* N = E.nodesA + nd;
             double bb[12];
 #pragma unroll
             for (int i = 0; i < 12; ++i) bb[i] = N->b[i];
             const int ch0 = N->c[0], ch1 = N->c[1];
             const bool o0 = box_overlap(bb, lo, hi), o1 = box_overlap(bb + 6, lo, hi);
-            if (o1) {
-              if (ch1 < 0) c1 = ~ch1;
-              else if (sp - bp < HCS_STACK) stk[(sp++) % HCS_STACK] = ch1;
-              else atomicOr(A.error, 1);
-            }
-            if (o0) {
-              if (ch0 < 0) c0 = ~ch0;
-              else if (sp - bp < HCS_STACK) stk[(sp++) % HCS_STACK] = ch0;
-              else atomicOr(A.error, 1);
+            /* push: a full LDS window first spills its bottom entry */
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+              const bool o = k == 0 ? o1 : o0;
+              const int ch = k == 0 ? ch1 : ch0;
+              if (!o) continue;
+              if (ch < 0) {
+                (k == 0 ? c1 : c0) = ~ch;
+                continue;
+              }
+              if (sp - bp == HCS_LSTACK) {
+                if (on == HCS_STACK - HCS_LSTACK) {
+                  atomicOr(A.error, 1);
+                  continue;
+                }
+                ovf[on++] = L.stk[(bp % HCS_LSTACK) * 64 + lane];
+                ++bp;
+              }
+              L.stk[(sp % HCS_LSTACK) * 64 + lane] = ch;
+              ++sp;
             }
           }
           const unsigned long long m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
@@ -819,8 +842,9 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
     A.item_ncand[it] = snc;
     A.item_nsurv[it] = nsurv;
     if (A.item_stats) {
-      unsigned long long* q = A.item_stats + 4 * (size_t)it;
+      unsigned long long* q = A.item_stats + 6 * (size_t)it;
       q[0] = clock64() - t_begin, q[1] = steps, q[2] = (unsigned long long)snc, q[3] = nsurv;
+      q[4] = w_begin, q[5] = wall_clock64();
     }
   }
 }

@@ -283,10 +283,11 @@ class Batch:
 
     def item_stats(self):
         """Per-item profile of one step: columns pair, kind, queries, cycles,
-        production steps, candidates, survivors, elements of B."""
+        production steps, candidates, survivors, elements of B, wall-clock
+        start and end."""
         n = C.c_int64()
         self.e._chk(self.e.lib.hcs_batch_item_stats(self.b, None, 0, C.byref(n)))
-        out = np.zeros((max(1, n.value), 8), dtype=np.int64)
+        out = np.zeros((max(1, n.value), 10), dtype=np.int64)
         self.e._chk(self.e.lib.hcs_batch_item_stats(self.b, A.i64ptr(out), out.shape[0], C.byref(n)))
         return out[:n.value]
 
