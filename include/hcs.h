@@ -135,7 +135,13 @@ int hcs_add_soft_mesh(hcs_ctx* ctx, const double* V, int32_t nv, const int32_t* 
                       const double* pressure, const hcs_props* props, int32_t geom_id, int32_t* handle);
 int hcs_add_rigid_mesh(hcs_ctx* ctx, const double* V, int32_t nv, const int32_t* F, int32_t ntri,
                        const hcs_props* props, int32_t geom_id, int32_t* handle);
-/* info = [kind (1 soft, 2 rigid), nv, elements, bvh height] */
+/* A MuJoCo plane geom: the rigid half-space z <= 0 of the geometry's frame,
+ * for the soft-plane branch of collision_cb (plugin.cpp:296-299, Drake's
+ * ComputeContactSurfaceFromSoftVolumeRigidHalfSpace).  Pairs (soft, plane)
+ * slice the soft tets by the plane z = 0; face keys are (pair, tet, 0, k). */
+int hcs_add_rigid_plane(hcs_ctx* ctx, const hcs_props* props, int32_t geom_id, int32_t* handle);
+/* info = [kind (1 soft, 2 rigid mesh, 3 rigid half-space), nv, elements,
+ * bvh height] */
 int hcs_mesh_info(hcs_ctx* ctx, int32_t handle, int64_t* info);
 /* Device tet records of a soft mesh, nt x 24 doubles (unit outward face
  * normals [12], plane offsets [4], pressure gradient [3], constant [1],

@@ -66,6 +66,7 @@ SIGNATURES = {
     "hcs_stream": (_P, [_P]),
     "hcs_add_soft_mesh": (C.c_int, [_P, _D, C.c_int32, _I32, C.c_int32, _D, C.POINTER(Props), C.c_int32, _I32]),
     "hcs_add_rigid_mesh": (C.c_int, [_P, _D, C.c_int32, _I32, C.c_int32, C.POINTER(Props), C.c_int32, _I32]),
+    "hcs_add_rigid_plane": (C.c_int, [_P, C.POINTER(Props), C.c_int32, _I32]),
     "hcs_mesh_info": (C.c_int, [_P, C.c_int32, _I64]),
     "hcs_mesh_tet_records": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_contact_step": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), C.POINTER(PairOut),

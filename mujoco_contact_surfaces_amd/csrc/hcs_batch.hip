@@ -136,7 +136,8 @@ static int batch_split(hcs_batch* b) {
     for (int i = 0; i < n; ++i) {
       const Item& it = b->plan.items[i];
       const EnvDesc& E = b->plan.envs[it.env];
-      if (E.kind == KIND_TET || E.ntA <= 1 || it.root < 0 || nc[i] <= b->split_target) continue;
+      if ((E.kind != KIND_TRI && E.kind != KIND_SOFT) || E.ntA <= 1 || it.root < 0 || nc[i] <= b->split_target)
+        continue;
       int d = 1;
       while (d < 6 && ((int64_t)b->split_target << d) < nc[i]) ++d;
       slot[i] = (int)req.size();

@@ -331,8 +331,8 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
   const double inf = __builtin_inf();
   double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
   if (lane < it.count) {
-    if (E.kind == KIND_TET) {
-      const double* b = E.boxA + 6 * (size_t)E.leafA[it.first + lane];
+    if (E.kind == KIND_TET || E.kind == KIND_PLANE) {
+      const double* b = E.boxLA + 6 * (size_t)(it.first + lane);
 #pragma unroll
       for (int c = 0; c < 3; ++c) lo[c] = b[c], hi[c] = b[3 + c];
     } else {
@@ -346,7 +346,10 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
     }
   }
   double t[6] = {inf, inf, inf, -inf, -inf, -inf};
-  if (E.kind == KIND_TET) {
+  if (E.kind == KIND_PLANE) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) t[c] = 0.0; /* the plane needs no target box */
+  } else if (E.kind == KIND_TET) {
     for (int v = lane; v < E.nvB; v += 64) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) t[c] = fmin(t[c], E.VB[3 * (size_t)v + c]), t[3 + c] = fmax(t[3 + c], E.VB[3 * (size_t)v + c]);
@@ -394,8 +397,14 @@ __global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, 
   if (k < nitems) {
     it = items[i];
     const double* bx = ibox + 12 * (size_t)i;
-    const bool tet = envs[it.env].kind == KIND_TET;
-    keep = boxes_may_overlap(xf + 12 * (size_t)it.env, tet ? bx + 6 : bx, tet ? bx : bx + 6);
+    const int kind = envs[it.env].kind;
+    const double* X = xf + 12 * (size_t)it.env;
+    if (kind == KIND_PLANE) {
+      keep = box_may_meet_plane(X, bx);
+    } else {
+      const bool tet = kind == KIND_TET;
+      keep = boxes_may_overlap(X, tet ? bx + 6 : bx, tet ? bx : bx + 6);
+    }
     if (!keep) ncand[i] = 0, nsurv[i] = 0;
     q = it.env % nq;
   }
@@ -914,7 +923,12 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
       E.mu = den == 0.0 ? 0.0 : 2 * m1 * m2 / den;
     }
     int nq, chunk;
-    if (E.b_soft) {
+    if (B.kind == 3) {
+      /* rigid half-space (plane geom): A's tets in Morton chunks */
+      E.kind = KIND_PLANE;
+      nq = A.nelem;
+      chunk = 64;
+    } else if (E.b_soft) {
       E.kind = KIND_SOFT;
       nq = B.nelem;
       chunk = per_item;
@@ -1067,6 +1081,21 @@ int hcs_add_rigid_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* F
     free_mesh(m);
     return rc;
   }
+  *handle = (int32_t)c->meshes.size();
+  c->meshes.push_back(m);
+  return HCS_OK;
+}
+
+int hcs_add_rigid_plane(hcs_ctx* h, const hcs_props* props, int32_t geom_id, int32_t* handle) {
+  if (!h || !props || !handle) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!is_rigid(*props)) return set_err(c, HCS_ERR_ARG, "rigid plane needs hydroelastic_modulus 0 or +inf");
+  Mesh m;
+  m.kind = 3;
+  m.geom_id = geom_id;
+  m.props = *props;
+  m.nv = 0;
+  m.nelem = 1; /* one element: the plane (candidate keys tet * 1 + 0) */
   *handle = (int32_t)c->meshes.size();
   c->meshes.push_back(m);
   return HCS_OK;

@@ -47,7 +47,10 @@ struct Node {
 };
 static_assert(sizeof(Node) == 128, "one LBVH node per cache line");
 
-enum EnvKind { KIND_TRI = 0, KIND_TET = 1, KIND_SOFT = 2 };
+/* query kinds: rigid triangles against A's LBVH, A's tets against a small
+ * rigid mesh, tets of a soft B against A's LBVH, A's tets against the rigid
+ * half-space z <= 0 of B's frame (a plane geom) */
+enum EnvKind { KIND_TRI = 0, KIND_TET = 1, KIND_SOFT = 2, KIND_PLANE = 3 };
 
 struct EnvDesc {
   const double* recA;

@@ -225,13 +225,122 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
 }
 
 /* One candidate (element a of A, element b of B). */
+/* Marching-tets slice of tet a of A by the plane s(x) = nh.x + off = 0
+ * (vertices with s > 0 are above): the 3- or 4-gon on the crossed edges,
+ * each point v_i + t (v_j - v_i), t = s_i / (s_i - s_j) along edge i < j,
+ * oriented counter-clockwise about nh.  False when the tet does not cross. */
+template <class T>
+__device__ inline bool slice_tet(const EnvDesc& E, int a, const T* nh, T off, Poly<T>& p) {
+  const int32_t* ta = E.TA + 4 * (size_t)a;
+  T vx[4], vy[4], vz[4], s[4];
+  int above = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double* v = E.VA + 3 * (size_t)ta[i];
+    vx[i] = T(v[0]), vy[i] = T(v[1]), vz[i] = T(v[2]);
+    T x[3] = {vx[i], vy[i], vz[i]};
+    s[i] = dot3(nh, x) + off;
+    above += val(s[i]) > 0;
+  }
+  if (above == 0 || above == 4) return false;
+  int up[4] = {0, 0, 0, 0}, dn[4] = {0, 0, 0, 0}, nu = 0, nd = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (val(s[i]) > 0) {
+      up[0] = (nu == 0) ? i : up[0];
+      up[1] = (nu == 1) ? i : up[1];
+      up[2] = (nu == 2) ? i : up[2];
+      ++nu;
+    } else {
+      dn[0] = (nd == 0) ? i : dn[0];
+      dn[1] = (nd == 1) ? i : dn[1];
+      dn[2] = (nd == 2) ? i : dn[2];
+      ++nd;
+    }
+  }
+  int ei[4], ej[4], ne;
+  if (nu == 1 || nd == 1) {
+    const int lone = nu == 1 ? up[0] : dn[0];
+    int q = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i != lone) {
+        const int lo = lone < i ? lone : i, hi = lone < i ? i : lone;
+        ei[0] = q == 0 ? lo : ei[0], ej[0] = q == 0 ? hi : ej[0];
+        ei[1] = q == 1 ? lo : ei[1], ej[1] = q == 1 ? hi : ej[1];
+        ei[2] = q == 2 ? lo : ei[2], ej[2] = q == 2 ? hi : ej[2];
+        ++q;
+      }
+    }
+    ei[3] = ej[3] = 0;
+    ne = 3;
+  } else {
+    const int pa[4] = {up[0], up[0], up[1], up[1]}, pb[4] = {dn[0], dn[1], dn[1], dn[0]};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ei[e] = pa[e] < pb[e] ? pa[e] : pb[e], ej[e] = pa[e] < pb[e] ? pb[e] : pa[e];
+    ne = 4;
+  }
+#pragma unroll
+  for (int i = 0; i < HCS_MAXV; ++i) p.x[i] = p.y[i] = p.z[i] = T(0.0);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (e < ne) {
+      T si = pick<T, 4>(s, ei[e]), sj = pick<T, 4>(s, ej[e]);
+      T t = si / (si - sj);
+      T xi = pick<T, 4>(vx, ei[e]), yi = pick<T, 4>(vy, ei[e]), zi = pick<T, 4>(vz, ei[e]);
+      T xj = pick<T, 4>(vx, ej[e]), yj = pick<T, 4>(vy, ej[e]), zj = pick<T, 4>(vz, ej[e]);
+      p.x[e] = xi + t * (xj - xi);
+      p.y[e] = yi + t * (yj - yi);
+      p.z[e] = zi + t * (zj - zi);
+    }
+  }
+  p.n = ne;
+  /* orient counter-clockwise about nh */
+  T W = T(0.0);
+#pragma unroll
+  for (int i = 1; i + 1 < 4; ++i) {
+    if (i + 1 < ne) {
+      T e1[3] = {p.x[i] - p.x[0], p.y[i] - p.y[0], p.z[i] - p.z[0]};
+      T e2[3] = {p.x[i + 1] - p.x[0], p.y[i + 1] - p.y[0], p.z[i + 1] - p.z[0]};
+      T cr[3];
+      cross3(e1, e2, cr);
+      W = W + dot3(cr, nh);
+    }
+  }
+  if (val(W) < 0) {
+    if (ne == 3) {
+      T tx = p.x[1], ty = p.y[1], tz = p.z[1];
+      p.x[1] = p.x[2], p.y[1] = p.y[2], p.z[1] = p.z[2];
+      p.x[2] = tx, p.y[2] = ty, p.z[2] = tz;
+    } else {
+      T tx = p.x[1], ty = p.y[1], tz = p.z[1];
+      p.x[1] = p.x[3], p.y[1] = p.y[3], p.z[1] = p.z[3];
+      p.x[3] = tx, p.y[3] = ty, p.z[3] = tz;
+    }
+  }
+  return true;
+}
+
 template <class T>
 __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic, int env, int a, int b, Acc<T>& acc,
                                          int64_t& np, int64_t& nf, const OutPtrs* o) {
   const double* __restrict__ rr = E.recA + (size_t)HCS_TREC * a;
   T gA[3] = {ic.escale * rr[16], ic.escale * rr[17], ic.escale * rr[18]};
   T cA = ic.escale * rr[19];
-  if (E.kind != KIND_SOFT) {
+  if (E.kind == KIND_PLANE) {
+    /* soft-half-space: B's plane z = 0 (normal +z, out of the rigid side)
+     * in A's frame; the gradient filter of rigid faces, then the slice */
+    const T ez[3] = {T(0.0), T(0.0), T(1.0)};
+    T nS[3];
+    rot(ic.X_AB, ez, nS);
+    if (!(((rr[20] * val(nS[0]) + rr[21] * val(nS[1])) + rr[22] * val(nS[2])) > HCS_COS_ALPHA)) return;
+    const T off = -dot3(nS, ic.X_AB + 9);
+    Poly<T> p;
+    if (!slice_tet<T>(E, a, nS, off, p)) return;
+    ++np;
+    T zero3[3] = {T(0.0), T(0.0), T(0.0)};
+    emit_polygon<T, 4>(ic, p, nS, gA, cA, zero3, acc, nf, o, env, a, b);
+  } else if (E.kind != KIND_SOFT) {
     /* soft-rigid: triangle b of the rigid mesh moved into A's frame */
     const int32_t* f = E.FB + 3 * (size_t)b;
     Poly<T> p;
@@ -272,94 +381,8 @@ __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic,
     if (!(((-val(nh[0]) * val(gbh[0]) + -val(nh[1]) * val(gbh[1])) + -val(nh[2]) * val(gbh[2])) > HCS_COS_ALPHA))
       return;
     /* marching-tets slice of tet a; above the plane is s > 0 */
-    const int32_t* ta = E.TA + 4 * (size_t)a;
-    T vx[4], vy[4], vz[4], s[4];
-    int above = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const double* v = E.VA + 3 * (size_t)ta[i];
-      vx[i] = T(v[0]), vy[i] = T(v[1]), vz[i] = T(v[2]);
-      T x[3] = {vx[i], vy[i], vz[i]};
-      s[i] = dot3(nh, x) + off;
-      above += val(s[i]) > 0;
-    }
-    if (above == 0 || above == 4) return;
-    int up[4] = {0, 0, 0, 0}, dn[4] = {0, 0, 0, 0}, nu = 0, nd = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (val(s[i]) > 0) {
-        up[0] = (nu == 0) ? i : up[0];
-        up[1] = (nu == 1) ? i : up[1];
-        up[2] = (nu == 2) ? i : up[2];
-        ++nu;
-      } else {
-        dn[0] = (nd == 0) ? i : dn[0];
-        dn[1] = (nd == 1) ? i : dn[1];
-        dn[2] = (nd == 2) ? i : dn[2];
-        ++nd;
-      }
-    }
-    int ei[4], ej[4], ne;
-    if (nu == 1 || nd == 1) {
-      const int lone = nu == 1 ? up[0] : dn[0];
-      int q = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (i != lone) {
-          const int lo = lone < i ? lone : i, hi = lone < i ? i : lone;
-          ei[0] = q == 0 ? lo : ei[0], ej[0] = q == 0 ? hi : ej[0];
-          ei[1] = q == 1 ? lo : ei[1], ej[1] = q == 1 ? hi : ej[1];
-          ei[2] = q == 2 ? lo : ei[2], ej[2] = q == 2 ? hi : ej[2];
-          ++q;
-        }
-      }
-      ei[3] = ej[3] = 0;
-      ne = 3;
-    } else {
-      const int pa[4] = {up[0], up[0], up[1], up[1]}, pb[4] = {dn[0], dn[1], dn[1], dn[0]};
-#pragma unroll
-      for (int e = 0; e < 4; ++e) ei[e] = pa[e] < pb[e] ? pa[e] : pb[e], ej[e] = pa[e] < pb[e] ? pb[e] : pa[e];
-      ne = 4;
-    }
     Poly<T> p;
-#pragma unroll
-    for (int i = 0; i < HCS_MAXV; ++i) p.x[i] = p.y[i] = p.z[i] = T(0.0);
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      if (e < ne) {
-        T si = pick<T, 4>(s, ei[e]), sj = pick<T, 4>(s, ej[e]);
-        T t = si / (si - sj);
-        T xi = pick<T, 4>(vx, ei[e]), yi = pick<T, 4>(vy, ei[e]), zi = pick<T, 4>(vz, ei[e]);
-        T xj = pick<T, 4>(vx, ej[e]), yj = pick<T, 4>(vy, ej[e]), zj = pick<T, 4>(vz, ej[e]);
-        p.x[e] = xi + t * (xj - xi);
-        p.y[e] = yi + t * (yj - yi);
-        p.z[e] = zi + t * (zj - zi);
-      }
-    }
-    p.n = ne;
-    /* orient counter-clockwise about nh */
-    T W = T(0.0);
-#pragma unroll
-    for (int i = 1; i + 1 < 4; ++i) {
-      if (i + 1 < ne) {
-        T e1[3] = {p.x[i] - p.x[0], p.y[i] - p.y[0], p.z[i] - p.z[0]};
-        T e2[3] = {p.x[i + 1] - p.x[0], p.y[i + 1] - p.y[0], p.z[i + 1] - p.z[0]};
-        T cr[3];
-        cross3(e1, e2, cr);
-        W = W + dot3(cr, nh);
-      }
-    }
-    if (val(W) < 0) {
-      if (ne == 3) {
-        T tx = p.x[1], ty = p.y[1], tz = p.z[1];
-        p.x[1] = p.x[2], p.y[1] = p.y[2], p.z[1] = p.z[2];
-        p.x[2] = tx, p.y[2] = ty, p.z[2] = tz;
-      } else {
-        T tx = p.x[1], ty = p.y[1], tz = p.z[1];
-        p.x[1] = p.x[3], p.y[1] = p.y[3], p.z[1] = p.z[3];
-        p.x[3] = tx, p.y[3] = ty, p.z[3] = tz;
-      }
-    }
+    if (!slice_tet<T>(E, a, nh, off, p)) return;
     /* tet b's half-spaces in A's frame */
     T pn[12], po[4];
 #pragma unroll
@@ -434,6 +457,19 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
 template <class T>
 __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b, const double* tv, const double* tn) {
   const double* r = E.recA + (size_t)HCS_TREC * a;
+  if (E.kind == KIND_PLANE) {
+    /* the gradient filter and the slice's sign test of process_candidate */
+    const double ez[3] = {0.0, 0.0, 1.0};
+    double nS[3];
+    rot(X, ez, nS);
+    if (!(((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA)) return false;
+    const double off = -dot3(nS, X + 9);
+    const int32_t* ta = E.TA + 4 * (size_t)a;
+    int above = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) above += (dot3(nS, E.VA + 3 * (size_t)ta[i]) + off) > 0;
+    return above != 0 && above != 4;
+  }
   if (E.kind != KIND_SOFT) {
     /* tv, tn: triangle b's vertices and unit normal already moved into A's
      * frame (xform / rot with X, the same arithmetic as process_candidate) */
@@ -516,6 +552,26 @@ __device__ inline bool boxes_may_overlap(const double* X, const double* P, const
   return true;
 }
 
+/* Whether a box Q (A's frame) can reach B's plane z = 0 placed in A's frame
+ * by X: |n.c + off| <= sum |n_k| h_k with a tolerance of 1e-9 of the
+ * coordinates involved, so an item it rejects has no tet straddling the
+ * plane under the exact per-tet test. */
+__device__ inline bool box_may_meet_plane(const double* X, const double* Q) {
+  const double ez[3] = {0.0, 0.0, 1.0};
+  double n[3];
+  rot(X, ez, n);
+  const double off = -dot3(n, X + 9);
+  double dist = off, rad = 0.0, mag = fabs(off);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double cq = 0.5 * (Q[c] + Q[3 + c]), hq = 0.5 * (Q[3 + c] - Q[c]);
+    dist += n[c] * cq;
+    rad += fabs(n[c]) * hq;
+    mag = fmax(mag, fabs(cq) + hq);
+  }
+  return !(fabs(dist) > rad + (1e-9 * mag + 1e-300));
+}
+
 /* ===================================================================
  * k_cand: candidate generation, exact cheap tests, survivor compaction
  * =================================================================== */
@@ -577,6 +633,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
   unsigned nsurv = 0;
   int q1n = 0, q2n = 0;
   const bool tet_mode = E.kind == KIND_TET;
+  const bool plane_mode = E.kind == KIND_PLANE;
   bool producing = true;
   /* traversal state (triangle / soft queries) */
   int ovf[HCS_STACK - HCS_LSTACK];
@@ -602,6 +659,30 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
       ulo[c] = wave_min(bx[c]);
       uhi[c] = wave_max(bx[3 + c]);
     }
+  } else if (plane_mode) {
+    /* 64 consecutive leaves; a tet is a candidate when its AABB straddles
+     * B's plane: |n.c + off| <= sum |n_k| h_k (centre c, half-extents h) */
+    valid = lane < item.count;
+    b = valid ? E.leafA[item.first + lane] : 0;
+    bool hit = false;
+    if (valid) {
+      const double* lb = E.boxLA + 6 * (size_t)(item.first + lane); /* = boxA of tet b */
+      const double ez[3] = {0.0, 0.0, 1.0};
+      double nS[3];
+      rot(X, ez, nS);
+      const double off = -dot3(nS, X + 9);
+      double dist = off, rad = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        dist += nS[c] * (0.5 * (lb[c] + lb[3 + c]));
+        rad += fabs(nS[c]) * (0.5 * (lb[3 + c] - lb[c]));
+      }
+      hit = !(fabs(dist) > rad);
+    }
+    const unsigned long long m = __ballot(hit);
+    if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, 0);
+    q1n += __popcll(m);
+    producing = false;
   } else {
     valid = lane < item.count;
     b = item.first + lane;
@@ -823,7 +904,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
             unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
             if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
           } else {
-            const int slot = tet_mode ? c.y - t0 : c.y - item.first;
+            const int slot = tet_mode ? c.y - t0 : (plane_mode ? 0 : c.y - item.first);
             keep = survives<double>(E, X, c.x, c.y, &L.tv[9 * slot], &L.tn[3 * slot]);
           }
         }

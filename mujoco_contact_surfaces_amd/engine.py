@@ -134,6 +134,14 @@ class Engine:
         self._kind[h.value] = "rigid"
         return h.value
 
+    def add_rigid_plane(self, props, geom_id):
+        """A MuJoCo plane geom: the rigid half-space z <= 0 of its frame."""
+        h = C.c_int32()
+        pr = _props(props)
+        self._chk(self.lib.hcs_add_rigid_plane(self.h, C.byref(pr), int(geom_id), C.byref(h)))
+        self._kind[h.value] = "rigid"
+        return h.value
+
     def mesh_info(self, handle):
         info = np.zeros(4, dtype=np.int64)
         self._chk(self.lib.hcs_mesh_info(self.h, int(handle), A.i64ptr(info)))
@@ -327,10 +335,17 @@ class ContactSurfacesPlugin:
         self.queue = []
         self.sensors = []
 
-    def add_geom(self, name, cs, V, elements, pressure=None):
+    def add_geom(self, name, cs, V=None, elements=None, pressure=None, plane=False):
+        """plane=True registers a MuJoCo plane geom (rigid half-space z <= 0
+        of its frame; plugin.cpp:296-299); V and elements are then unused."""
         props = props_from_cs(cs)
         gid = len(self.geoms)
-        if props["hydroelastic_modulus"] > 0:
+        if plane:
+            if props["hydroelastic_modulus"] > 0:
+                raise ValueError("a plane geom is rigid only (plugin.cpp:634-647)")
+            h = self.engine.add_rigid_plane(props, gid)
+            kind = "rigid"
+        elif props["hydroelastic_modulus"] > 0:
             h = self.engine.add_soft_mesh(V, elements, pressure, props, gid)
             kind = "soft"
         else:

@@ -274,3 +274,29 @@ def soft_soft_env(seed, env, nt_range=(2e4, 2e5)):
                              geom_id=gid)
     return dict(soft=mk(bodies[0], 2 * env), soft_b=mk(bodies[1], 2 * env + 1),
                 X_WA=pose(RA, [0, 0, 0]), X_WB=pose(RB, cB), V_WA=twist(rng), V_WB=twist(rng))
+
+
+def soft_plane_env(seed, env, nt_range=(2e3, 3e4), geom_order=0):
+    """A soft body resting on a MuJoCo ground plane (rigid half-space z <= 0
+    of a tilted frame): the soft-plane branch of collision_cb."""
+    rng = SplitMix64(BASE_SEED, 6, seed, env)
+    nt_target = rng.loguniform(*nt_range)
+    L, K = choose_LK(nt_target)
+    E = rng.loguniform(3e4, 3e5)
+    if rng.choice(2) == 0:
+        r = rng.uniform(0.03, 0.06)
+        V, T, p = soft_mesh(L, K, 0, r, r, r, E)
+    else:
+        V, T, p = soft_mesh(L, K, 1, rng.uniform(0.02, 0.06), rng.uniform(0.02, 0.06), rng.uniform(0.02, 0.06), E)
+    R = random_rotation(rng)
+    RP = axis_angle(random_axis(rng), math.radians(rng.uniform(0, 20)))  # the plane's frame
+    n = RP[:, 2]
+    pen = rng.uniform(0.5e-3, 6e-3)
+    cP = np.array([rng.uniform(-0.1, 0.1), rng.uniform(-0.1, 0.1), rng.uniform(-0.05, 0.05)])
+    VW = V @ R.T
+    lowest = float((VW @ n).min())
+    c = cP - (lowest + pen) * n
+    gid_s, gid_p = (2 * env, 2 * env + 1) if geom_order == 0 else (2 * env + 1, 2 * env)
+    return dict(soft=dict(V=V, T=T, p=p, props=dict(hydroelastic_modulus=E, **SOFT_PROPS), geom_id=gid_s),
+                plane=dict(props=dict(RIGID_PROPS), geom_id=gid_p),
+                X_WA=pose(R, c), X_WB=pose(RP, cP), V_WA=twist(rng), V_WB=twist(rng, 0.1, 0.005))

@@ -542,6 +542,57 @@ void finish_wrench(const Wrench& w, double* out) {
   }
 }
 
+/* Marching-tets slice of one tet by the plane s(x) = nh.x + off = 0
+ * (vertices with s > 0 are above): the 3- or 4-gon on the crossed edges,
+ * each point v_i + t (v_j - v_i) with t = s_i / (s_i - s_j) along the edge
+ * (i < j), oriented counter-clockwise about nh.  Returns its vertex count,
+ * 0 when the tet does not cross the plane. */
+int slice_tet(const double* V, const int32_t* tet, const double* nh, double off, double (*poly)[3]) {
+  const double* v[4];
+  double s[4];
+  int above = 0;
+  for (int i = 0; i < 4; ++i) {
+    v[i] = V + 3 * tet[i];
+    s[i] = dot3(nh, v[i]) + off;
+    above += s[i] > 0;
+  }
+  if (above == 0 || above == 4) return 0;
+  int np = 0;
+  int up[4], dn[4], nu = 0, nd = 0;
+  for (int i = 0; i < 4; ++i) (s[i] > 0 ? up[nu++] : dn[nd++]) = i;
+  int edges[4][2];
+  int ne = 0;
+  if (nu == 1 || nd == 1) {
+    int lone = nu == 1 ? up[0] : dn[0];
+    for (int i = 0; i < 4; ++i)
+      if (i != lone) edges[ne][0] = std::min(lone, i), edges[ne][1] = std::max(lone, i), ++ne;
+  } else {
+    /* cyclic order (u0,d0) (u0,d1) (u1,d1) (u1,d0) */
+    int pr[4][2] = {{up[0], dn[0]}, {up[0], dn[1]}, {up[1], dn[1]}, {up[1], dn[0]}};
+    for (int e = 0; e < 4; ++e) edges[e][0] = std::min(pr[e][0], pr[e][1]), edges[e][1] = std::max(pr[e][0], pr[e][1]);
+    ne = 4;
+  }
+  for (int e = 0; e < ne; ++e) {
+    int i = edges[e][0], j = edges[e][1];
+    double t = s[i] / (s[i] - s[j]);
+    for (int c = 0; c < 3; ++c) poly[np][c] = v[i][c] + t * (v[j][c] - v[i][c]);
+    ++np;
+  }
+  /* Orient counter-clockwise about nh. */
+  double W = 0;
+  for (int i = 1; i + 1 < np; ++i) {
+    double e1[3], e2[3], cr[3];
+    sub3(poly[i], poly[0], e1);
+    sub3(poly[i + 1], poly[0], e2);
+    cross3(e1, e2, cr);
+    W += dot3(cr, nh);
+  }
+  if (W < 0)
+    for (int i = 1, j = np - 1; i < j; ++i, --j)
+      for (int c = 0; c < 3; ++c) std::swap(poly[i][c], poly[j][c]);
+  return np;
+}
+
 }  // namespace
 
 extern "C" {
@@ -932,7 +983,6 @@ int orc_soft_soft(const double* VA, int nva, const int32_t* TA, int nta, const d
 
   Wrench w;
   Outputs o{faces, face_keys, face_cap, polys, poly_keys, poly_cap};
-  const int kEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
   for (int64_t key : keys) {
     int a = (int)(key / ntb), b = (int)(key % ntb);
     const double* ra = &recA[(size_t)ORC_TREC * a];
@@ -950,51 +1000,9 @@ int orc_soft_soft(const double* VA, int nva, const int32_t* TA, int nta, const d
     if (!(dot3(nh, ra + 20) > kCosAlpha)) continue;
     double mnh[3] = {-nh[0], -nh[1], -nh[2]};
     if (!(dot3(mnh, gbh) > kCosAlpha)) continue;
-    /* Slice tet a (marching tets); vertices above the plane have s > 0. */
-    const double* v[4];
-    double s[4];
-    int above = 0;
-    for (int i = 0; i < 4; ++i) {
-      v[i] = VA + 3 * TA[4 * a + i];
-      s[i] = dot3(nh, v[i]) + off;
-      above += s[i] > 0;
-    }
-    if (above == 0 || above == 4) continue;
     double poly[8][3];
-    int np = 0;
-    int up[4], dn[4], nu = 0, nd = 0;
-    for (int i = 0; i < 4; ++i) (s[i] > 0 ? up[nu++] : dn[nd++]) = i;
-    int edges[4][2];
-    int ne = 0;
-    if (nu == 1 || nd == 1) {
-      int lone = nu == 1 ? up[0] : dn[0];
-      for (int i = 0; i < 4; ++i)
-        if (i != lone) edges[ne][0] = std::min(lone, i), edges[ne][1] = std::max(lone, i), ++ne;
-    } else {
-      /* cyclic order (u0,d0) (u0,d1) (u1,d1) (u1,d0) */
-      int pr[4][2] = {{up[0], dn[0]}, {up[0], dn[1]}, {up[1], dn[1]}, {up[1], dn[0]}};
-      for (int e = 0; e < 4; ++e) edges[e][0] = std::min(pr[e][0], pr[e][1]), edges[e][1] = std::max(pr[e][0], pr[e][1]);
-      ne = 4;
-    }
-    (void)kEdge;
-    for (int e = 0; e < ne; ++e) {
-      int i = edges[e][0], j = edges[e][1];
-      double t = s[i] / (s[i] - s[j]);
-      for (int c = 0; c < 3; ++c) poly[np][c] = v[i][c] + t * (v[j][c] - v[i][c]);
-      ++np;
-    }
-    /* Orient counter-clockwise about nh. */
-    double W = 0;
-    for (int i = 1; i + 1 < np; ++i) {
-      double e1[3], e2[3], cr[3];
-      sub3(poly[i], poly[0], e1);
-      sub3(poly[i + 1], poly[0], e2);
-      cross3(e1, e2, cr);
-      W += dot3(cr, nh);
-    }
-    if (W < 0)
-      for (int i = 1, j = np - 1; i < j; ++i, --j)
-        for (int c = 0; c < 3; ++c) std::swap(poly[i][c], poly[j][c]);
+    const int np = slice_tet(VA, TA + 4 * a, nh, off, poly);
+    if (np == 0) continue;
     /* Tet b's half-spaces in A's frame. */
     double pn[12], po[4];
     for (int k = 0; k < 4; ++k) {
@@ -1009,6 +1017,66 @@ int orc_soft_soft(const double* VA, int nva, const int32_t* TA, int nta, const d
   }
   finish_wrench(w, wrench);
   counts[0] = (int64_t)keys.size();
+  counts[1] = o.npoly;
+  counts[2] = o.nface;
+  return 0;
+}
+
+/* collision_cb SOFT-PLANE branch (plugin.cpp:296-299): Drake semantics of
+ * ComputeContactSurfaceFromSoftVolumeRigidHalfSpace.  The rigid geometry
+ * is the half-space z <= 0 of its frame; its boundary plane in the soft
+ * frame has unit normal n = R_SR e_z (out of the rigid body, into the soft
+ * one) through p_SR.  Candidates: tets whose AABB straddles the plane
+ * (|n.c + off| <= sum |n_k| h_k for centre c and half-extents h).  Each
+ * candidate passes the gradient filter of rigid faces (n.unit(grad p) >
+ * cos 5pi/8), is sliced by the plane (marching tets) and its polygon is
+ * evaluated with the soft field; keys (tet, 0). */
+int orc_soft_plane(const double* VS, int nvs, const int32_t* T, int nt, const double* pS, const double* pose,
+                   const double* params, double* wrench, int64_t* counts, double* faces, int64_t* face_keys,
+                   int64_t face_cap, double* polys, int64_t* poly_keys, int64_t poly_cap) {
+  std::vector<double> rec((size_t)ORC_TREC * nt), box;
+  orc_tet_precompute(VS, nvs, T, nt, pS, rec.data());
+  tet_boxes(rec, nt, box);
+  PairCtx pc;
+  pc.X_WA = pose;
+  pc.X_WB = pose + 12;
+  pc.V_WA = pose + 24;
+  pc.V_WB = pose + 30;
+  pc.dstar = orc_combined_dissipation(params[0], params[1], kInf, 1.0);
+  pc.mu = orc_combined_friction(params[2], params[3]);
+  pc.dt = params[4];
+  pc.apply = params[5] != 0;
+  pc.repr = (int)params[6];
+  pc.a_is_M = params[7] != 0;
+  double X_SR[12];
+  orc_compose_SR(pc.X_WA, pc.X_WB, X_SR);
+  const double ez[3] = {0, 0, 1};
+  double n[3];
+  rot(X_SR, ez, n);
+  const double off = -dot3(n, X_SR + 9);
+  Wrench w;
+  Outputs o{faces, face_keys, face_cap, polys, poly_keys, poly_cap};
+  const double zero3[3] = {0, 0, 0};
+  int64_t ncand = 0;
+  for (int t = 0; t < nt; ++t) {
+    const double* b = &box[6 * (size_t)t];
+    double dist = off, rad = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      dist += n[c] * (0.5 * (b[c] + b[3 + c]));
+      rad += std::fabs(n[c]) * (0.5 * (b[3 + c] - b[c]));
+    }
+    if (std::fabs(dist) > rad) continue;
+    ++ncand;
+    const double* r = &rec[(size_t)ORC_TREC * t];
+    /* Drake semantics (IsPlaneNormalAlongPressureGradient), as for faces */
+    if (!(dot3(r + 20, n) > kCosAlpha)) continue;
+    double poly[8][3];
+    const int np = slice_tet(VS, T + 4 * t, n, off, poly);
+    if (np == 0) continue;
+    emit_polygon(pc, poly, np, n, r, zero3, false, t, 0, w, o);
+  }
+  finish_wrench(w, wrench);
+  counts[0] = ncand;
   counts[1] = o.npoly;
   counts[2] = o.nface;
   return 0;

@@ -118,6 +118,13 @@ int orc_soft_soft(const double* VA, int nva, const int32_t* TA, int nta, const d
                   double* faces, int64_t* face_keys, int64_t face_cap,
                   double* polys, int64_t* poly_keys, int64_t poly_cap);
 
+/* Soft body vs the rigid half-space z <= 0 of frame B (a MuJoCo plane
+ * geom): params = [E_A, d_A, mu_A, mu_B, dt, apply, representation,
+ * a_is_M]; outputs as orc_soft_rigid, keys (tet, 0). */
+int orc_soft_plane(const double* VS, int nvs, const int32_t* T, int nt, const double* pS, const double* pose,
+                   const double* params, double* wrench, int64_t* counts, double* faces, int64_t* face_keys,
+                   int64_t face_cap, double* polys, int64_t* poly_keys, int64_t poly_cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,8 @@ def lib():
                                      _I64, _D, _I64, C.c_int64, _D, _I64, C.c_int64]
         L.orc_soft_soft.argtypes = [_D, C.c_int, _I32, C.c_int, _D, _D, C.c_int, _I32, C.c_int, _D, _D, _D, C.c_int,
                                     _D, _I64, _D, _I64, C.c_int64, _D, _I64, C.c_int64]
+        L.orc_soft_plane.argtypes = [_D, C.c_int, _I32, C.c_int, _D, _D, _D, _D, _I64, _D, _I64, C.c_int64, _D,
+                                     _I64, C.c_int64]
         L.orc_soft_prepare.restype = C.c_void_p
         L.orc_soft_prepare.argtypes = [_D, C.c_int, _I32, C.c_int, _D]
         L.orc_soft_free.argtypes = [C.c_void_p]
@@ -144,6 +146,30 @@ def soft_rigid(VS, T, pS, VR, F, X_WS, X_WR, V_WS, V_WR, E, d, mu_s, mu_r, repre
                          dp(pose), dp(params), int(brute), dp(wrench), lp(counts), dp(faces), lp(fk), cap[0],
                          dp(polys), lp(pk), cap[1])
     nf, npl = int(counts[2]), int(counts[1])
+    return dict(wrench=wrench, counts=counts, faces=faces[:nf], face_keys=fk[:nf], polys=polys[:npl],
+                poly_keys=pk[:npl])
+
+
+def soft_plane(VS, T, pS, X_WS, X_WR, V_WS, V_WR, E, d, mu_s, mu_r, representation=0, apply=True, soft_is_M=True,
+               dt=0.001, want=True):
+    """Soft body vs the rigid half-space z <= 0 of frame R (a MuJoCo plane
+    geom; plugin.cpp:296-299).  Returns dict(wrench, counts, faces,
+    face_keys, polys, poly_keys); keys are (tet, 0, ...)."""
+    VS, T, pS = _d(VS), _i(T), _d(pS)
+    pose = _d(np.concatenate([X_WS, X_WR, V_WS, V_WR]))
+    params = _d([E, d, mu_s, mu_r, dt, int(apply), representation, int(soft_is_M)])
+    wrench = np.zeros(12)
+    counts = np.zeros(3, dtype=np.int64)
+    args = (dp(VS), VS.shape[0], ip(T), T.shape[0], dp(pS), dp(pose), dp(params), dp(wrench), lp(counts))
+    lib().orc_soft_plane(*args, None, None, 0, None, None, 0)
+    if not want:
+        return dict(wrench=wrench, counts=counts)
+    nf, npl = int(counts[2]), int(counts[1])
+    faces = np.zeros((nf + 1, FREC))
+    fk = np.zeros((nf + 1, 3), dtype=np.int64)
+    polys = np.zeros((npl + 1, PREC))
+    pk = np.zeros((npl + 1, 2), dtype=np.int64)
+    lib().orc_soft_plane(*args, dp(faces), lp(fk), nf + 1, dp(polys), lp(pk), npl + 1)
     return dict(wrench=wrench, counts=counts, faces=faces[:nf], face_keys=fk[:nf], polys=polys[:npl],
                 poly_keys=pk[:npl])
 

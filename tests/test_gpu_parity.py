@@ -405,3 +405,72 @@ def test_soft_soft_jacobian(engine):
         assert np.abs(J[:, d] - col).max() <= 1e-6 * scale + 1e-9 * np.abs(W).max(), (d, J[:, d], col)
         checked += 1
     assert checked >= 3
+
+
+# ----------------------------------------------------------- rigid half-space
+
+
+def _add_plane_scene(engine, sc):
+    s = sc["soft"]
+    ha = engine.add_soft_mesh(s["V"], s["T"], s["p"], s["props"], s["geom_id"])
+    hb = engine.add_rigid_plane(sc["plane"]["props"], sc["plane"]["geom_id"])
+    return ha, hb
+
+
+def _oracle_plane(sc, representation=0):
+    s = sc["soft"]
+    return O.soft_plane(s["V"], s["T"], s["p"], sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"],
+                        s["props"]["hydroelastic_modulus"], s["props"]["dissipation"], s["props"]["dynamic_friction"],
+                        sc["plane"]["props"]["dynamic_friction"], representation=representation,
+                        soft_is_M=s["geom_id"] < sc["plane"]["geom_id"])
+
+
+@pytest.mark.parametrize("seed,order", [(0, 0), (1, 1), (2, 0)])
+@pytest.mark.parametrize("representation", [0, 1])
+def test_soft_plane_parity(engine, seed, order, representation):
+    say("soft-plane parity", seed, order, representation)
+    sc = S.soft_plane_env(seed, 0, geom_order=order)
+    ha, hb = _add_plane_scene(engine, sc)
+    ref = _oracle_plane(sc, representation)
+    assert ref["counts"][1] > 0
+    W, cnt, (rec, key) = engine.contact_step([pair_of(sc, ha, hb)], representation, face_capacity=1 << 16)
+    np.testing.assert_array_equal(cnt[0], ref["counts"])
+    assert_wrench_close(W[0], ref["wrench"])
+    match_faces(rec, key, ref["faces"], ref["face_keys"])
+    keys = engine.candidates(pair_of(sc, ha, hb))
+    assert keys.size == ref["counts"][0] and np.unique(keys).size == keys.size
+    assert keys.min() >= 0 and keys.max() < sc["soft"]["T"].shape[0]
+
+
+def test_batched_with_planes_matches_plain_and_oracle(engine):
+    say("batched with planes")
+    scs, handles = [], []
+    for e in range(6):
+        sc = S.soft_plane_env(7, e, geom_order=e % 2)
+        scs.append(sc)
+        handles.append(_add_plane_scene(engine, sc))
+    sm, hs, st = _small_batch(engine, n=6, seed=8)
+    state = np.array([np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]]) for sc in scs])
+    b = engine.batch(np.array(handles + hs, dtype=np.int32))
+    b.set_state(np.concatenate([state, st]))
+    b.step()
+    W, cnt = b.get()
+    for i, (sc, (ha, hb)) in enumerate(zip(scs, handles)):
+        ref = _oracle_plane(sc)
+        np.testing.assert_array_equal(cnt[i], ref["counts"])
+        assert_wrench_close(W[i], ref["wrench"])
+        Wp, cp = engine.contact_step([pair_of(sc, ha, hb)])
+        np.testing.assert_array_equal(cnt[i], cp[0])
+        assert_wrench_close(W[i], Wp[0], rtol=1e-12)
+    for k, (sc, (ha, hb)) in enumerate(zip(sm, hs)):
+        ref = oracle_soft_rigid(sc)
+        np.testing.assert_array_equal(cnt[len(scs) + k], ref["counts"])
+        assert_wrench_close(W[len(scs) + k], ref["wrench"])
+    say("batched with planes, split and tuned")
+    b.set_config(4, 1, 0, 16)
+    b.step()
+    W2, cnt2 = b.get()
+    np.testing.assert_array_equal(cnt2, cnt)
+    for i in range(cnt.shape[0]):
+        assert_wrench_close(W2[i], W[i], rtol=1e-12)
+    b.close()

@@ -416,3 +416,84 @@ def test_soft_from_surface_centroid_fan():
     np.testing.assert_allclose(Vo[-1], [0.01, -0.02, 0.03], atol=1e-15)
     assert signed_volumes(Vo, To).min() > 0
     assert po[-1] == 2e4 and np.all(po[:-1] == 0)
+
+
+# ----------------------------------------------------------- rigid half-space
+
+
+@pytest.mark.parametrize("h", [0.37, -0.5, 0.0])
+def test_half_space_section_area_and_force(h):
+    V, T, p = _grid_scene()
+    # rigid half-space z <= h: its plane cuts the cube in a 2x2 square
+    res = O.soft_plane(V, T, p, pose(), pose(t=(0, 0, h)), np.zeros(6), np.zeros(6), 1e3, 0.0, 0.3, 0.3,
+                       representation=1)
+    assert res["counts"][1] > 0
+    assert res["faces"][:, 9].sum() == pytest.approx(4.0, rel=1e-12)
+    # kPolygon faces: p0 = p(centroid) = 1000 (h + 1) everywhere on the plane
+    np.testing.assert_allclose(res["faces"][:, 10], 1000 * (h + 1), rtol=1e-12)
+    np.testing.assert_allclose(res["wrench"][:3], [0, 0, 4.0 * 1000 * (h + 1)], rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(res["wrench"][6:9], -res["wrench"][:3], rtol=0, atol=0)
+    # keys: (tet, 0)
+    assert (res["face_keys"][:, 1] == 0).all() and (res["poly_keys"][:, 1] == 0).all()
+
+
+def test_half_space_tilted_plane_closed_form():
+    V, T, p = _grid_scene()
+    h, ang = 0.37, 0.2
+    R = rot([1, 0, 0], ang)
+    n = R[:, 2]
+    res = O.soft_plane(V, T, p, pose(), pose(R, (0, 0, h)), np.zeros(6), np.zeros(6), 1e3, 0.0, 0.3, 0.3)
+    # the section over the cube's (x, y) square: area 4 / n_z, force
+    # 1000 * 4 (h + 1) / n_z along n (the odd terms of z(x, y) integrate to 0)
+    assert res["faces"][:, 9].sum() == pytest.approx(4.0 / n[2], rel=1e-12)
+    np.testing.assert_allclose(res["wrench"][:3], n * 4000.0 * (h + 1) / n[2], rtol=1e-12, atol=1e-9)
+
+
+def test_half_space_normal_against_the_gradient_is_filtered():
+    V, T, p = _grid_scene()
+    # plane normal -z (rigid side above): opposed to the pressure gradient
+    res = O.soft_plane(V, T, p, pose(), pose(rot([1, 0, 0], np.pi), (0, 0, 0.2)), np.zeros(6), np.zeros(6), 1e3,
+                       0.0, 0.3, 0.3)
+    assert res["counts"][0] > 0 and res["counts"][1] == 0 and res["counts"][2] == 0
+
+
+def test_half_space_candidates_are_the_tets_straddling_the_plane():
+    V, T, p = _grid_scene(n=3)
+    R = rot([0.3, -0.2, 1.0], 0.4)
+    X_WR = pose(R, (0.1, -0.05, 0.21))
+    res = O.soft_plane(V, T, p, pose(), X_WR, np.zeros(6), np.zeros(6), 1e3, 0.0, 0.3, 0.3)
+    # independent count: AABB of each tet against the plane, in numpy
+    n, c = R[:, 2], X_WR[9:]
+    lo, hi = V[T].min(axis=1), V[T].max(axis=1)
+    mid, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    dist = (mid - c) @ n
+    rad = np.abs(half) @ np.abs(n)
+    assert res["counts"][0] == int((np.abs(dist) <= rad).sum())
+    # polygons only where vertices lie on both sides
+    s = (V[T] - c) @ n
+    crossing = ((s > 0).sum(axis=1) % 4) != 0
+    assert res["counts"][1] == int(crossing.sum())
+
+
+def test_half_space_equals_a_plane_grid_mesh_on_force_and_area():
+    from mujoco_contact_surfaces_amd import scenes as S
+    V, T, p = S.soft_mesh(2, 3, 0, 0.05, 0.05, 0.05, 2e5)
+    X_WS = pose(t=(0.0, 0.0, 0.045))
+    VS = np.array([0, 0, 0, 0.01, -0.02, -0.05])  # translation only: vn is the same at every point
+    # a large grid of rigid triangles in the plane z = 0 (normal +z)
+    m, half = 9, 0.2
+    xs = np.linspace(-half, half, m)
+    g = np.array([[x, y, 0.0] for y in xs for x in xs])
+    F = []
+    for j in range(m - 1):
+        for i in range(m - 1):
+            a = j * m + i
+            F += [[a, a + 1, a + m + 1], [a, a + m + 1, a + m]]
+    F = np.array(F, np.int32)
+    for repr_ in (0, 1):
+        hs = O.soft_plane(V, T, p, X_WS, pose(), VS, np.zeros(6), 2e5, 3.0, 0.4, 0.3, representation=repr_)
+        gr = O.soft_rigid(V, T, p, g, F, X_WS, pose(), VS, np.zeros(6), 2e5, 3.0, 0.4, 0.3, representation=repr_)
+        assert hs["counts"][1] > 0
+        assert hs["faces"][:, 9].sum() == pytest.approx(gr["faces"][:, 9].sum(), rel=1e-12)
+        np.testing.assert_allclose(hs["wrench"][:3], gr["wrench"][:3], rtol=1e-11,
+                                   atol=1e-12 * np.abs(gr["wrench"][:3]).max())
