@@ -77,6 +77,19 @@ def algorithmic_bytes(engine, scs, handles, counts, survivors, nitems):
                                                     polygons=int(counts[:, 1].sum()))
 
 
+def measured_traffic(kernel, envs, seed):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), when
+    it covers this workload; else (None, None)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_batched_4096env.json")
+    if envs != 4096 or seed != 0 or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    v = d.get("hbm_bytes_per_launch", {}).get(kernel)
+    return (v, d.get("source")) if v is not None else (None, None)
+
+
 def cpu_baseline(scs, budget_s=12.0, chunk=128):
     """Oracle per-step work (load-time BVH excluded) on the environments in
     order, chunk by chunk, until about budget_s of serial step time: each
@@ -202,9 +215,10 @@ def main():
     dom = "k_cand" if kt["cand"] >= kt["clip"] else "k_clip"
     k_ms = kt["cand"] if dom == "k_cand" else kt["clip"]
     achieved = B[dom] / (k_ms * 1e-3)
+    traffic, tsrc = measured_traffic(dom, a.envs, a.seed)
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, kernel=dom, kernel_ms=k_ms, alg_bytes=B[dom], kernel_ms_all=kt,
-                alg_bytes_all=B, **bparts)
+                traffic=traffic, traffic_source=tsrc, kernel=dom, kernel_ms=k_ms, alg_bytes=B[dom],
+                kernel_ms_all=kt, alg_bytes_all=B, **bparts)
     cpu = None if a.no_cpu else cpu_baseline(scs)
     value = clips / dt
     out = dict(metric=METRIC, value=value, unit="clips/s", n_gpus=1, steps=a.steps, warmup=a.warmup,

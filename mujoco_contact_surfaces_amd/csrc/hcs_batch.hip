@@ -95,13 +95,16 @@ static int batch_upload(hcs_batch* b) {
  * new plan (at most 4 rounds) while some item still exceeds it.  The
  * frontier's subtrees hold each leaf exactly once, so the candidate set is
  * unchanged. */
-/* One eager step of the current plan; per-item candidate counts into nc. */
+/* Per-item candidate counts of the current plan on the current state. */
 static int batch_measure(hcs_batch* b, std::vector<int>& nc) {
   Ctx* c = b->c;
   const int n = (int)b->plan.items.size();
   nc.assign(n, 0);
   if (n == 0) return HCS_OK;
-  batch_enqueue(b, batch_args(b));
+  /* counts only: no survivors, no clipping */
+  const QueryArgs a = batch_args(b);
+  pipe_front(c, b->pipe, a, b->npairs);
+  launch_cand(c->stream, a, grid_of(b), CAND_COUNT);
   int rc = hip_check(c, hipGetLastError(), "measure step");
   if (!rc)
     rc = hip_check(c, hipMemcpyAsync(nc.data(), b->pipe.ncand.ptr, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream),
@@ -418,7 +421,7 @@ int hcs_batch_kernel_times(hcs_batch* b, int32_t reps, double* ms) {
     Pipe& P = b->pipe;
     pipe_front(c, P, a, b->npairs);
     (void)hipEventRecord(ev[0], s);
-    launch_cand(s, a, grid_of(b));
+    launch_cand(s, a, grid_of(b), CAND_STEP);
     (void)hipEventRecord(ev[1], s);
     launch_clip_reduce(c, P, a, b->npairs, clip_grid_for(c, P.cap), ev[2], ev[4]);
     (void)hipEventRecord(ev[3], s);

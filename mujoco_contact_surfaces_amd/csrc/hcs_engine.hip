@@ -460,7 +460,7 @@ __global__ void k_split_roots(const EnvDesc* __restrict__ envs, const Item* __re
 /* Candidate generation: one wave per item; persistent when a work-queue
  * counter is given (the counter is reset by a fill kernel in the same
  * graph). */
-template <bool KEYS>
+template <int MODE>
 __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
   unsigned long long s_next = 0, s_end = 0; /* this wave's survivor slots */
@@ -474,9 +474,9 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
       if (v < n) {
         if (A.live) {
           const int4 r = A.live_item[A.qseg[x] + v];
-          run_cand<KEYS>(L, A, A.live[A.qseg[x] + v], Item{r.x, r.y, r.z, r.w}, s_next, s_end);
+          run_cand<MODE>(L, A, A.live[A.qseg[x] + v], Item{r.x, r.y, r.z, r.w}, s_next, s_end);
         } else {
-          run_cand<KEYS>(L, A, (int)v, A.items[v], s_next, s_end);
+          run_cand<MODE>(L, A, (int)v, A.items[v], s_next, s_end);
         }
         break;
       }
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
           nrec = A.live_item[base + vn];
           if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
         }
-        run_cand<KEYS>(L, A, idx, Item{rec.x, rec.y, rec.z, rec.w}, s_next, s_end);
+        run_cand<MODE>(L, A, idx, Item{rec.x, rec.y, rec.z, rec.w}, s_next, s_end);
         if (vn >= n) break;
         idx = nidx;
         rec = nrec;
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
       if (lane == 0) v = atomicAdd(A.counter, 1u);
       v = __shfl(v, 0, 64);
       if (v >= (unsigned)A.nitems) break;
-      run_cand<KEYS>(L, A, (int)v, A.items[v], s_next, s_end);
+      run_cand<MODE>(L, A, (int)v, A.items[v], s_next, s_end);
     }
   }
   fill_sentinels(A, s_next, s_end);
@@ -652,12 +652,14 @@ void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs) {
                        a.item_ncand, a.item_nsurv);
 }
 
-void launch_cand(hipStream_t s, const QueryArgs& a, int grid) {
+void launch_cand(hipStream_t s, const QueryArgs& a, int grid, int mode) {
   if (a.nitems <= 0 || grid <= 0) return;
-  if (a.cand_key)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<true>), dim3(grid), dim3(64), 0, s, a);
+  if (mode == CAND_KEYS)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<CAND_KEYS>), dim3(grid), dim3(64), 0, s, a);
+  else if (mode == CAND_COUNT)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<CAND_COUNT>), dim3(grid), dim3(64), 0, s, a);
   else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<false>), dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<CAND_STEP>), dim3(grid), dim3(64), 0, s, a);
 }
 
 /* Scan of the per-item survivor counts, [event e_clip], clip (double, or
@@ -688,7 +690,7 @@ static void pipe_back(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_
 void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid, int clip_grid, int ndir,
                   const double* dstar_dE, const double* invE, bool cand_only, int dir0) {
   pipe_front(c, p, a, npairs);
-  launch_cand(c->stream, a, cand_grid);
+  launch_cand(c->stream, a, cand_grid, a.cand_key ? CAND_KEYS : CAND_STEP);
   if (cand_only) return;
   pipe_back(c, p, a, npairs, clip_grid, ndir, dstar_dE, invE, dir0, nullptr, nullptr);
 }

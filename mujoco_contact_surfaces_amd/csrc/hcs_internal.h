@@ -110,7 +110,8 @@ void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs);
 int split_roots(Ctx* c, const EnvDesc* envs, const Item* items, const std::vector<int>& req,
                 const std::vector<int>& depth, std::vector<int32_t>& out);
 void pipe_free(Pipe& p);
-void launch_cand(hipStream_t s, const QueryArgs& a, int grid);
+/* mode: CandMode */
+void launch_cand(hipStream_t s, const QueryArgs& a, int grid, int mode);
 void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, hipEvent_t e_clip,
                         hipEvent_t e_reduce);
 int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan);

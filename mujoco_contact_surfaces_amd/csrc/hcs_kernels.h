@@ -52,6 +52,10 @@ static_assert(sizeof(Node) == 128, "one LBVH node per cache line");
  * half-space z <= 0 of B's frame (a plane geom) */
 enum EnvKind { KIND_TRI = 0, KIND_TET = 1, KIND_SOFT = 2, KIND_PLANE = 3 };
 
+/* candidate kernel modes: a contact step (exact rejections, survivors), the
+ * canonical candidate keys, or per-item candidate counts only (planning) */
+enum CandMode { CAND_STEP = 0, CAND_KEYS = 1, CAND_COUNT = 2 };
+
 struct EnvDesc {
   const double* recA;
   const double* boxA;

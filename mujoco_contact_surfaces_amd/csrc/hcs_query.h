@@ -614,9 +614,9 @@ __device__ inline void fill_sentinels(const QueryArgs& A, unsigned long long fro
  * tests on every 64 queued candidates, and append survivors to HBM in
  * groups of up to 64 with a per-item sequence number, so the clip kernel can
  * place each contribution at item_off[item] + seq whatever order the groups
- * land in.  KEYS: write the canonical candidate keys instead (no tests, no
- * survivors). */
-template <bool KEYS>
+ * land in.  CAND_KEYS writes the canonical candidate keys instead and
+ * CAND_COUNT only counts candidates per item (no tests, no survivors). */
+template <int MODE>
 __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const Item item,
                                 unsigned long long& s_next, unsigned long long& s_end) {
   const int lane = threadIdx.x & 63;
@@ -900,10 +900,10 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
         if (lane < take) {
           c = L.q1[base + lane];
           ++nc;
-          if (KEYS) {
+          if (MODE == CAND_KEYS) {
             unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
             if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
-          } else {
+          } else if (MODE == CAND_STEP) {
             const int slot = tet_mode ? c.y - t0 : (plane_mode ? 0 : c.y - item.first);
             keep = survives<double>(E, X, c.x, c.y, &L.tv[9 * slot], &L.tn[3 * slot]);
           }
