@@ -529,7 +529,7 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
 /* Clip, faces and force of every survivor (grid-stride); grid.y selects the
  * tangent direction in the Dual instantiation. */
 template <class T>
-__global__ __launch_bounds__(256) void k_clip(QueryArgs A, ClipArgs C, int dir_base, Contrib* out,
+__global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int dir_base, Contrib* out,
                                               int64_t out_stride) {
   const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
                   A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan};

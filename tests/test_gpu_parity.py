@@ -474,3 +474,36 @@ def test_batched_with_planes_matches_plain_and_oracle(engine):
     for i in range(cnt.shape[0]):
         assert_wrench_close(W2[i], W[i], rtol=1e-12)
     b.close()
+
+
+def test_batched_soft_soft_matches_plain_and_oracle(engine):
+    """Soft-soft pairs in one batched step (the 256-pair soft-soft form, at
+    sizes the oracle finishes quickly), split and re-planned."""
+    say("batched soft-soft")
+    scs = [S.soft_soft_env(11, e, nt_range=(2e3, 8e3)) for e in range(4)]
+    handles = [add_scene(engine, sc) for sc in scs]
+    state = np.array([np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]]) for sc in scs])
+    b = engine.batch(np.array(handles, dtype=np.int32))
+    b.set_state(state)
+    b.step()
+    W, cnt = b.get()
+    for i, (sc, (ha, hb)) in enumerate(zip(scs, handles)):
+        a_, b_ = sc["soft"], sc["soft_b"]
+        ref = O.soft_soft(a_["V"], a_["T"], a_["p"], b_["V"], b_["T"], b_["p"], sc["X_WA"], sc["X_WB"], sc["V_WA"],
+                          sc["V_WB"], a_["props"]["hydroelastic_modulus"], a_["props"]["dissipation"],
+                          a_["props"]["dynamic_friction"], b_["props"]["hydroelastic_modulus"],
+                          b_["props"]["dissipation"], b_["props"]["dynamic_friction"],
+                          a_is_M=a_["geom_id"] < b_["geom_id"], want=False)
+        np.testing.assert_array_equal(cnt[i], ref["counts"])
+        assert_wrench_close(W[i], ref["wrench"])
+        Wp, cp = engine.contact_step([pair_of(sc, ha, hb)])
+        np.testing.assert_array_equal(cnt[i], cp[0])
+        assert_wrench_close(W[i], Wp[0], rtol=1e-12)
+    assert cnt[:, 1].sum() > 0
+    b.set_config(8, 1, 0, 64)
+    b.step()
+    W2, cnt2 = b.get()
+    np.testing.assert_array_equal(cnt2, cnt)
+    for i in range(len(scs)):
+        assert_wrench_close(W2[i], W[i], rtol=1e-12)
+    b.close()
