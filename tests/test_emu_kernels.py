@@ -1,0 +1,192 @@
+"""The engine's candidate and clip device code (hcs_query.h) compiled for the
+CPU (tests/emu): each work item runs on an emulated 64-lane wave (threads
+meeting at every ballot, shuffle and barrier), then every survivor is
+clipped in double and, per tangent direction, in the Dual scalar type.
+
+Checked here, on the CPU: counts bit-exact and wrenches within 1e-10 of the
+oracle for triangle, tet, soft-soft and half-space queries and for items
+split over LBVH subtrees; no item that the per-step cull rejects has a
+candidate; and the tangent pass against central finite differences of the
+oracle within 1e-6 (the differentiation path, switched off on the GPU).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mujoco_contact_surfaces_amd import scenes as S
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+KTRI, KTET, KSOFT, KPLANE = 0, 1, 2, 3
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emu") / "libhcs_emu.so")
+    cmd = ["g++", "-O2", "-std=c++20", "-ffp-contract=off", "-fPIC", "-shared", "-Wno-unknown-pragmas",
+           "-I", os.path.join(ROOT, "tests", "emu", "include"),
+           "-I", os.path.join(ROOT, "mujoco_contact_surfaces_amd", "csrc"), "-I", os.path.join(ROOT, "oracle"),
+           os.path.join(ROOT, "tests", "emu", "emu_kernels.cpp"), os.path.join(ROOT, "oracle", "hcs_oracle.cpp"),
+           "-o", out, "-lpthread"]
+    subprocess.run(cmd, check=True, timeout=600)
+    lib = C.CDLL(out)
+    lib.emu_pair.restype = C.c_int
+    lib.emu_pair.argtypes = [_D, C.c_int, _I32, C.c_int, _D, _D, C.c_int, _I32, C.c_int, _D, C.c_int, _D, _D,
+                             C.c_int, C.c_int, _D, _D, _I64, _I64]
+    return lib
+
+
+def _f(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def run_pair(emu, sc, kind, X_WB=None, E_A=None, split=0, ndir=0, representation=0):
+    s = sc["soft"]
+    EA = s["props"]["hydroelastic_modulus"] if E_A is None else E_A
+    VA, TA, pA = _f(s["V"]), np.ascontiguousarray(s["T"], np.int32), _f(s["p"]) * (EA / s["props"]["hydroelastic_modulus"])
+    pB = None
+    if kind == KPLANE:
+        other = sc["plane"]
+        VB, FB, nb = np.zeros((1, 3)), np.zeros((1, 3), np.int32), 1
+        EB, dB = np.inf, 1.0
+    elif kind == KSOFT:
+        other = sc["soft_b"]
+        VB, FB, pB = _f(other["V"]), np.ascontiguousarray(other["T"], np.int32), _f(other["p"])
+        nb = FB.shape[0]
+        EB, dB = other["props"]["hydroelastic_modulus"], other["props"]["dissipation"]
+    else:
+        other = sc["rigid"]
+        VB, FB = _f(other["V"]), np.ascontiguousarray(other["F"], np.int32)
+        nb = FB.shape[0]
+        EB, dB = np.inf, 1.0
+    XB = sc["X_WB"] if X_WB is None else X_WB
+    state = _f(np.concatenate([sc["X_WA"], XB, sc["V_WA"], sc["V_WB"]]))
+    params = _f([EA, s["props"]["dissipation"], EB, dB, s["props"]["dynamic_friction"],
+                 other["props"]["dynamic_friction"], 0.001, 1, representation, int(s["geom_id"] < other["geom_id"])])
+    dsE = 0.0 if kind != KSOFT else EB * (dB - s["props"]["dissipation"]) / (EA + EB) ** 2
+    seeds = _f([dsE, 1.0 / EA])
+    W = np.zeros((1 + ndir, 12))
+    counts = np.zeros(3, np.int64)
+    stats = np.zeros(5, np.int64)
+    rc = emu.emu_pair(_dp(VA), VA.shape[0], TA.ctypes.data_as(_I32), TA.shape[0], _dp(pA), _dp(VB), VB.shape[0],
+                      FB.ctypes.data_as(_I32), nb, _dp(pB) if pB is not None else None, kind, _dp(state),
+                      _dp(params), split, ndir, _dp(seeds), _dp(W), counts.ctypes.data_as(_I64),
+                      stats.ctypes.data_as(_I64))
+    assert rc == 0
+    return W, counts, stats
+
+
+def oracle(sc, kind, X_WB=None, E_A=None, representation=0):
+    s = sc["soft"]
+    EA = s["props"]["hydroelastic_modulus"] if E_A is None else E_A
+    p = s["p"] * (EA / s["props"]["hydroelastic_modulus"])
+    XB = sc["X_WB"] if X_WB is None else X_WB
+    if kind == KPLANE:
+        return O.soft_plane(s["V"], s["T"], p, sc["X_WA"], XB, sc["V_WA"], sc["V_WB"], EA, s["props"]["dissipation"],
+                            s["props"]["dynamic_friction"], sc["plane"]["props"]["dynamic_friction"],
+                            representation=representation, soft_is_M=s["geom_id"] < sc["plane"]["geom_id"],
+                            want=False)
+    if kind == KSOFT:
+        b = sc["soft_b"]
+        return O.soft_soft(s["V"], s["T"], p, b["V"], b["T"], b["p"], sc["X_WA"], XB, sc["V_WA"], sc["V_WB"], EA,
+                           s["props"]["dissipation"], s["props"]["dynamic_friction"], b["props"]["hydroelastic_modulus"],
+                           b["props"]["dissipation"], b["props"]["dynamic_friction"], representation=representation,
+                           a_is_M=s["geom_id"] < b["geom_id"], want=False)
+    r = sc["rigid"]
+    return O.soft_rigid(s["V"], s["T"], p, r["V"], r["F"], sc["X_WA"], XB, sc["V_WA"], sc["V_WB"], EA,
+                        s["props"]["dissipation"], s["props"]["dynamic_friction"], r["props"]["dynamic_friction"],
+                        representation=representation, soft_is_M=s["geom_id"] < r["geom_id"], want=False)
+
+
+def assert_wrench_close(got, ref, rtol=1e-10):
+    scale = np.linalg.norm(ref[:3]) + 1e-300
+    for b in range(4):
+        g, r = got[3 * b:3 * b + 3], ref[3 * b:3 * b + 3]
+        assert np.linalg.norm(g - r) <= rtol * np.linalg.norm(r) + 1e-14 * scale, (b, g, r)
+
+
+def _scene(kind):
+    if kind == KTRI:
+        return S.differentiation_env(0, 1)
+    if kind == KTET:
+        return S.sphere_box(1)
+    if kind == KSOFT:
+        return S.soft_soft_env(2, 0, nt_range=(2e3, 4e3))
+    return S.soft_plane_env(3, 0, nt_range=(2e3, 6e3), geom_order=1)
+
+
+@pytest.mark.parametrize("kind,split", [(KTRI, 0), (KTRI, 3), (KTET, 0), (KSOFT, 0), (KSOFT, 2), (KPLANE, 0)])
+def test_kernel_bodies_match_the_oracle(emu, kind, split):
+    sc = _scene(kind)
+    W, counts, stats = run_pair(emu, sc, kind, split=split, representation=split % 2)
+    ref = oracle(sc, kind, representation=split % 2)
+    assert ref["counts"][1] > 0
+    np.testing.assert_array_equal(counts, ref["counts"])
+    assert_wrench_close(W[0], ref["wrench"])
+    assert stats[2] == 0, "an item rejected by its static boxes has candidates"
+    assert stats[4] == 0, "traversal error"
+    if split:
+        assert stats[1] > 0  # subtree items away from the contact are culled
+
+
+def _rotated(X, d, h):
+    a = np.zeros(3)
+    a[d] = h
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    th = np.linalg.norm(a)
+    Rd = np.eye(3) + np.sin(th) / th * K + (1 - np.cos(th)) / th ** 2 * K @ K
+    Y = X.copy()
+    Y[:9] = (Rd @ X[:9].reshape(3, 3)).reshape(9)
+    return Y
+
+
+def _perturbed(X, d, h):
+    if d < 3:
+        return _rotated(X, d, h)
+    Y = X.copy()
+    Y[9 + d - 3] += h
+    return Y
+
+
+@pytest.mark.parametrize("kind", [KTRI, KTET, KPLANE, KSOFT])
+def test_tangent_pass_matches_central_differences(emu, kind):
+    sc = _scene(kind)
+    W, counts, _ = run_pair(emu, sc, kind, ndir=7)
+    base = oracle(sc, kind)
+    np.testing.assert_array_equal(counts, base["counts"])
+    h = 1e-7
+    checked = 0
+    for d in range(6):
+        rp = oracle(sc, kind, X_WB=_perturbed(sc["X_WB"], d, h))
+        rm = oracle(sc, kind, X_WB=_perturbed(sc["X_WB"], d, -h))
+        if not (np.array_equal(rp["counts"][1:], base["counts"][1:]) and
+                np.array_equal(rm["counts"][1:], base["counts"][1:])):
+            continue  # the clipping topology (polygons, faces) changes within the step
+        fd = (rp["wrench"] - rm["wrench"]) / (2 * h)
+        J = W[1 + d]
+        scale = np.abs(J).max() + 1e-300
+        # 1e-6 of the column, plus the roundoff of the central difference
+        # itself: eps * sum|face terms| / h, a few times eps |W| / h = 1.1e-9 |W|
+        # per unit pose step at h = 1e-7 once hundreds of faces are summed
+        assert np.abs(J - fd).max() <= 1e-6 * scale + 5e-8 * np.abs(W[0]).max(), (d, J, fd)
+        checked += 1
+    assert checked >= 4
+    # d/dE_A: pressures scale with E_A; against a rigid B nothing else does
+    E = sc["soft"]["props"]["hydroelastic_modulus"]
+    if kind != KSOFT:
+        np.testing.assert_allclose(W[7], W[0] / E, rtol=1e-9, atol=1e-12 * np.abs(W[0]).max() / E)
+    else:
+        hE = 1e-7 * E
+        fd = (oracle(sc, kind, E_A=E + hE)["wrench"] - oracle(sc, kind, E_A=E - hE)["wrench"]) / (2 * hE)
+        assert np.abs(W[7] - fd).max() <= 1e-6 * np.abs(W[7]).max() + 1e-12 * np.abs(W[0]).max() / E
