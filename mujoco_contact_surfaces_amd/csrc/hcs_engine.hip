@@ -337,7 +337,8 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
       for (int c = 0; c < 3; ++c) lo[c] = b[c], hi[c] = b[3 + c];
     } else {
       const int nvq = E.kind == KIND_SOFT ? 4 : 3;
-      const int32_t* f = E.FB + (size_t)nvq * (it.first + lane);
+      const int q = it.first + lane;
+      const int32_t* f = E.FB + (size_t)nvq * (E.qorder ? E.qorder[q] : q);
       for (int k = 0; k < nvq; ++k) {
         const double* v = E.VB + 3 * (size_t)f[k];
 #pragma unroll
@@ -870,7 +871,7 @@ static int build_bvh(Ctx* c, Mesh& m) {
 
 static void free_mesh(Mesh& m) {
   for (void* p : {(void*)m.V, (void*)m.E, (void*)m.p, (void*)m.rec, (void*)m.box, (void*)m.nodes, (void*)m.nrm,
-                  (void*)m.leaf, (void*)m.boxL})
+                  (void*)m.leaf, (void*)m.boxL, (void*)m.qorder})
     if (p) (void)hipFree(p);
   m = Mesh();
 }
@@ -897,6 +898,7 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
     E.ntA = A.nelem;
     E.xtri_off = -1;
     E.VB = B.V, E.FB = B.E, E.nrmB = B.nrm, E.recB = B.rec, E.nB = B.nelem, E.nvB = B.nv;
+    E.qorder = B.kind == 2 ? B.qorder : (B.kind == 1 ? B.leaf : nullptr);
     E.b_soft = B.kind == 1;
     E.a_is_M = A.geom_id < B.geom_id;
     E.repr = o ? o->representation : 0;
@@ -1073,6 +1075,38 @@ int hcs_add_rigid_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* F
   m.nelem = ntri;
   int rc = upload(c, (void**)&m.V, V, 3 * sizeof(double) * nv);
   if (rc == HCS_OK) rc = upload(c, (void**)&m.E, F, 3 * sizeof(int32_t) * ntri);
+  if (rc == HCS_OK) {
+    /* query order: triangles sorted by the Morton code of their centroids,
+     * so a work item's 64 triangles are spatially compact */
+    std::vector<double> cen(3 * (size_t)ntri);
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int32_t t = 0; t < ntri; ++t)
+      for (int k = 0; k < 3; ++k) {
+        const double x = (V[3 * F[3 * t] + k] + V[3 * F[3 * t + 1] + k] + V[3 * F[3 * t + 2] + k]) / 3.0;
+        cen[3 * (size_t)t + k] = x;
+        lo[k] = std::min(lo[k], x), hi[k] = std::max(hi[k], x);
+      }
+    auto spread = [](uint32_t v) {
+      v = (v * 0x00010001u) & 0xFF0000FFu;
+      v = (v * 0x00000101u) & 0x0F00F00Fu;
+      v = (v * 0x00000011u) & 0xC30C30C3u;
+      v = (v * 0x00000005u) & 0x49249249u;
+      return v;
+    };
+    std::vector<uint32_t> code(ntri);
+    for (int32_t t = 0; t < ntri; ++t) {
+      uint32_t q[3];
+      for (int k = 0; k < 3; ++k) {
+        const double u = hi[k] > lo[k] ? (cen[3 * (size_t)t + k] - lo[k]) / (hi[k] - lo[k]) * 1023.0 : 0.0;
+        q[k] = (uint32_t)std::min(1023.0, std::max(0.0, u));
+      }
+      code[t] = (spread(q[0]) << 2) | (spread(q[1]) << 1) | spread(q[2]);
+    }
+    std::vector<int32_t> order(ntri);
+    for (int32_t t = 0; t < ntri; ++t) order[t] = t;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
+    rc = upload(c, (void**)&m.qorder, order.data(), sizeof(int32_t) * (size_t)ntri);
+  }
   if (rc == HCS_OK) rc = hip_check(c, hipMalloc((void**)&m.nrm, 3 * sizeof(double) * (size_t)ntri), "hipMalloc");
   if (rc == HCS_OK) {
     hipLaunchKernelGGL(k_tri_normals, dim3((ntri + 127) / 128), dim3(128), 0, c->stream, m.V, m.E, ntri, m.nrm);

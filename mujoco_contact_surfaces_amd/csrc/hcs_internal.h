@@ -25,6 +25,7 @@ struct Mesh {
   double* box = nullptr;
   Node* nodes = nullptr;
   int32_t* leaf = nullptr; /* tets in Morton (BVH leaf) order */
+  int32_t* qorder = nullptr; /* rigid: triangles in Morton order of their centroids */
   double* boxL = nullptr;   /* their AABBs in that order */
   double* nrm = nullptr;
 };

@@ -68,6 +68,8 @@ struct EnvDesc {
   const int32_t* FB;  /* rigid B triangles, or soft B tets */
   const double* nrmB; /* rigid B normals */
   const double* recB; /* soft B records */
+  const int32_t* qorder; /* triangle / soft queries: B's elements in a
+                            spatially compact order (nullptr: identity) */
   int32_t ntA, nB, kind, nvB;
   int32_t a_is_M, repr, apply, b_soft;
   double mu, dstar, dt;

@@ -595,6 +595,7 @@ struct CandLds {
   double box[64 * 6];
   double tv[64 * 9]; /* query / tile triangles in A's frame */
   double tn[64 * 3]; /* their unit normals in A's frame */
+  int qb[64];        /* triangle / soft queries: the element of each slot */
   int2 xfer[64];
   double X[12];
 };
@@ -684,11 +685,16 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
     q1n += __popcll(m);
     producing = false;
   } else {
+    /* b is the lane's query slot; its element comes through qorder (a
+     * spatially compact order of B's elements), and candidates carry the
+     * slot until the exact rejections turn it into the element */
     valid = lane < item.count;
-    b = item.first + lane;
+    b = lane;
+    const int e = E.qorder ? E.qorder[item.first + lane < E.nB ? item.first + lane : 0] : item.first + lane;
+    L.qb[lane] = e;
     if (valid) {
       const int nvq = E.kind == KIND_SOFT ? 4 : 3;
-      const int32_t* f = E.FB + (size_t)nvq * b;
+      const int32_t* f = E.FB + (size_t)nvq * e;
       for (int i = 0; i < nvq; ++i) {
         double y[3];
         xform(X, E.VB + 3 * (size_t)f[i], y);
@@ -704,7 +710,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
       }
       if (E.kind != KIND_SOFT) {
         double nS[3];
-        rot(X, E.nrmB + 3 * (size_t)b, nS);
+        rot(X, E.nrmB + 3 * (size_t)e, nS);
 #pragma unroll
         for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
       }
@@ -800,7 +806,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
             if (donor) {
               const int r = (int)lane_prefix(donors);
               if (r < npair) {
-                L.xfer[r] = make_int2(b - item.first, L.stk[(bp % HCS_LSTACK) * 64 + lane]);
+                L.xfer[r] = make_int2(b, L.stk[(bp % HCS_LSTACK) * 64 + lane]);
                 ++bp;
                 --cnt;
               }
@@ -810,7 +816,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
               const int r = (int)lane_prefix(idle);
               if (r < npair) {
                 const int2 w = L.xfer[r];
-                b = item.first + w.x;
+                b = w.x;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) lo[c] = L.box[6 * w.x + c], hi[c] = L.box[6 * w.x + 3 + c];
                 L.stk[lane] = w.y;
@@ -900,11 +906,14 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
         if (lane < take) {
           c = L.q1[base + lane];
           ++nc;
+          /* element of B and the LDS slot of its moved triangle */
+          const bool by_slot = !tet_mode && !plane_mode;
+          const int slot = tet_mode ? c.y - t0 : (plane_mode ? 0 : c.y);
+          c.y = by_slot ? L.qb[c.y] : c.y;
           if (MODE == CAND_KEYS) {
-            unsigned long long slot = atomicAdd(A.cand_cnt, 1ull);
-            if ((int64_t)slot < A.cand_cap) A.cand_key[slot] = (int64_t)c.x * E.nB + c.y;
+            unsigned long long k = atomicAdd(A.cand_cnt, 1ull);
+            if ((int64_t)k < A.cand_cap) A.cand_key[k] = (int64_t)c.x * E.nB + c.y;
           } else if (MODE == CAND_STEP) {
-            const int slot = tet_mode ? c.y - t0 : (plane_mode ? 0 : c.y - item.first);
             keep = survives<double>(E, X, c.x, c.y, &L.tv[9 * slot], &L.tn[3 * slot]);
           }
         }

@@ -163,6 +163,11 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
   E.dstar = orc_combined_dissipation(params[0], params[1], params[2], params[3]);
   E.mu = orc_combined_friction(params[4], params[5]);
   E.xtri_off = -1;
+  /* query order of B's elements: reversed, so slot -> element mapping is
+   * exercised (the engine uses a Morton order) */
+  std::vector<int32_t> qorder(std::max(1, nb));
+  for (int i = 0; i < nb; ++i) qorder[i] = nb - 1 - i;
+  E.qorder = (kind == KIND_TRI || kind == KIND_SOFT) ? qorder.data() : nullptr;
   /* items and their LBVH roots */
   const bool chunks_of_A = kind == KIND_TET || kind == KIND_PLANE;
   const int nq = chunks_of_A ? nta : nb;
@@ -195,7 +200,7 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
         for (int c = 0; c < 3; ++c) q[c] = std::fmin(q[c], b[c]), q[3 + c] = std::fmax(q[3 + c], b[3 + c]);
       } else {
         const int nvq = kind == KIND_SOFT ? 4 : 3;
-        const int32_t* f = FB + nvq * (size_t)(it.first + l);
+        const int32_t* f = FB + nvq * (size_t)qorder[it.first + l];
         for (int k = 0; k < nvq; ++k)
           for (int c = 0; c < 3; ++c)
             q[c] = std::fmin(q[c], VB[3 * f[k] + c]), q[3 + c] = std::fmax(q[3 + c], VB[3 * f[k] + c]);
