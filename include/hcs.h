@@ -114,6 +114,25 @@ typedef struct hcs_taxel_grid {
   double resolution;           /* taxel pitch */
 } hcs_taxel_grid;
 
+/* TaxelSensor (S/include/mujoco_contact_surface_sensors/taxel_sensor.h,
+ * S/src/taxel_sensor.cpp:46-156): taxel positions in the sensor geometry's
+ * frame (any mesh-geom offset of taxel_sensor.cpp:108-139 already applied),
+ * the rosparam values include_margin, sample_resolution, method and
+ * sample_method, and the visualize flag, which the reference's closest
+ * method depends on. */
+typedef struct hcs_taxel_sensor {
+  int32_t geom;          /* mesh handle of the sensor geometry */
+  int32_t ntaxels;
+  int32_t method;        /* 0 closest, 1 weighted, 2 mean, 3 squared */
+  int32_t sample_method; /* 0 default, 1 area_importance */
+  int32_t visualize;
+  int32_t reserved;
+  double include_margin;
+  double sample_resolution;
+  double X_W_sensor[12]; /* sensor geom pose */
+  const double* taxels;  /* ntaxels x 3, host memory */
+} hcs_taxel_sensor;
+
 /* ---- lifetime (plugin constructor/load/destructor, plugin.cpp:191-245) */
 int hcs_create(hcs_ctx** ctx, int device_ordinal);
 /* Also destroys the context's remaining batches; their handles become
@@ -165,6 +184,11 @@ int hcs_candidates(hcs_ctx* ctx, const hcs_pair_in* pair, int64_t* keys, int64_t
  * (S/src/flat_tactile_sensor.cpp:196-197). */
 int hcs_taxel_dims(const hcs_taxel_grid* g, int32_t* cx, int32_t* cy);
 int hcs_taxels(hcs_ctx* ctx, int32_t nsensors, const hcs_taxel_grid* g, double* out);
+/* TaxelSensor::internal_update (S/src/taxel_sensor.cpp:158-478) on the
+ * polygons of the last hcs_contact_step run with keep_polygons = 1.
+ * values (ntaxels, in/out): taxels no sample reaches keep their value, as
+ * the reference's message does; *nsamples (optional) = surface samples. */
+int hcs_taxel_sensor_update(hcs_ctx* ctx, const hcs_taxel_sensor* sensor, double* values, int64_t* nsamples);
 
 /* ---- batched form (no reference counterpart: the reference runs one pair
  * at a time inside mj_collision).  Thousands of independent pairs, one

@@ -52,6 +52,13 @@ class TaxelGrid(C.Structure):
                 ("half_size", C.c_double * 3), ("resolution", C.c_double)]
 
 
+class TaxelSensor(C.Structure):
+    _fields_ = [("geom", C.c_int32), ("ntaxels", C.c_int32), ("method", C.c_int32), ("sample_method", C.c_int32),
+                ("visualize", C.c_int32), ("reserved", C.c_int32), ("include_margin", C.c_double),
+                ("sample_resolution", C.c_double), ("X_W_sensor", C.c_double * 12),
+                ("taxels", C.POINTER(C.c_double))]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
@@ -74,6 +81,7 @@ SIGNATURES = {
     "hcs_candidates": (C.c_int, [_P, C.POINTER(PairIn), _I64, C.c_int64, _I64]),
     "hcs_taxel_dims": (C.c_int, [C.POINTER(TaxelGrid), _I32, _I32]),
     "hcs_taxels": (C.c_int, [_P, C.c_int32, C.POINTER(TaxelGrid), _D]),
+    "hcs_taxel_sensor_update": (C.c_int, [_P, C.POINTER(TaxelSensor), _D, _I64]),
     "hcs_batch_create": (C.c_int, [_P, C.c_int32, _I32, C.POINTER(StepOpts), C.POINTER(_P)]),
     "hcs_batch_destroy": (C.c_int, [_P]),
     "hcs_batch_set_state": (C.c_int, [_P, _D]),

@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libhcs.so")
 
-HIP_SOURCES = ["hcs_engine.hip", "hcs_batch.hip", "hcs_taxels.hip", "hcs_adjoint.hip"]
+HIP_SOURCES = ["hcs_engine.hip", "hcs_batch.hip", "hcs_taxels.hip", "hcs_taxel_sensor.hip", "hcs_adjoint.hip"]
 CXX_SOURCES = ["hcs_gen.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1009,6 +1009,10 @@ int hcs_destroy(hcs_ctx* h) {
                   &c->d_cand, &c->d_cand_cnt, &c->d_tx_cnt, &c->d_tx_off, &c->d_tx_list, &c->d_tx_out,
                   &c->d_tx_grid})
     if (b->ptr) (void)hipFree(b->ptr);
+  TaxelSensorBufs& T = c->ts;
+  for (DBuf* b : {&T.mask, &T.k64, &T.kp, &T.idx, &T.cnt, &T.nfan, &T.fan_off, &T.tmp, &T.tris, &T.surf, &T.scnt, &T.soff,
+                  &T.P, &T.E, &T.taxel, &T.val})
+    if (b->ptr) (void)hipFree(b->ptr);
   (void)hipStreamDestroy(c->stream);
   delete h;
   return HCS_OK;

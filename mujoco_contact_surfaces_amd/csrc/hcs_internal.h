@@ -64,6 +64,11 @@ struct Plan {
   std::vector<int> order; /* items by decreasing measured work (empty: item order) */
 };
 
+/* Scratch of hcs_taxel_sensor_update. */
+struct TaxelSensorBufs {
+  DBuf mask, k64, kp, idx, cnt, nfan, fan_off, tmp, tris, surf, scnt, soff, P, E, taxel, val;
+};
+
 struct Ctx {
   int device = 0;
   int num_cu = 0;
@@ -77,6 +82,7 @@ struct Ctx {
   DBuf d_poly_rec, d_poly_key, d_poly_cnt;
   DBuf d_cand, d_cand_cnt;
   DBuf d_tx_cnt, d_tx_off, d_tx_list, d_tx_out, d_tx_grid;
+  TaxelSensorBufs ts;
   /* last contact step's polygons (for taxels) */
   int64_t last_npoly = 0;
   int last_valid = 0;

@@ -1,0 +1,378 @@
+/*
+ * hcs_taxel_sensor.hip -- the taxel sensor on the GPU, FP64.
+ *
+ * Restates TaxelSensor::internal_update
+ * (mujoco_contact_surface_sensors/src/taxel_sensor.cpp:158-478) over the
+ * contact polygons kept by the last contact step:
+ *   1. the polygons of the pairs touching the sensor geometry, in key order
+ *      (pair, element of A, element of B) -- two stable radix passes -- and
+ *      their kTriangle fans (v_k, v_{k+1}, C), the surface's triangles;
+ *   2. samples on every triangle: DEFAULT (:174-203), the reference's loops
+ *      a, b += 1/st from 0 while <= 1 with st from |v1 - v0| and |v2 - v0|
+ *      (both st1 and st2 from v2 - v0), one thread per triangle after a
+ *      count and a scan; or AREA_IMPORTANCE (:205-246), a single sequential
+ *      stream of libstdc++'s default_random_engine through
+ *      uniform_real_distribution<double>(0, 1), one thread;
+ *   3. one wavefront per taxel: dist = (-2 t.p + |t|^2) + |p|^2 to every
+ *      sample; CLOSEST (:285-319) takes the first nearest sample, and within
+ *      the margin writes its pressure when visualize is set and |p| > 1e-6,
+ *      else 0; WEIGHTED and MEAN fall through to SQUARED (:320-457, no
+ *      break): sum of max(0, margin - sqrt(dist))^2 |p| over the samples in
+ *      the margin, times sample_resolution.  A taxel no sample reaches keeps
+ *      its value; with no sample at all every taxel is 0 (:458-477).
+ * Sums over samples run lane-strided in sample order and then in a fixed
+ * butterfly, so results are deterministic.
+ */
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hcs_device.h"
+#include "hcs_internal.h"
+#include "hcs_query.h"
+
+#pragma clang fp contract(off)
+
+using namespace hcs;
+
+namespace hcs {
+
+#define HCS_TS_TRI 12 /* fan triangle: v0, v1, v2 (W), e0, e1, e2 */
+
+/* selected polygons get their key, others sort last */
+__global__ void k_ts_keys(const int64_t* __restrict__ key, int64_t np, const unsigned char* __restrict__ mask,
+                          unsigned long long* __restrict__ k64, unsigned* __restrict__ kp, int* __restrict__ idx,
+                          unsigned* __restrict__ cnt) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int64_t pair = key[3 * i], a = key[3 * i + 1], b = key[3 * i + 2];
+  const bool on = mask[pair] != 0;
+  k64[i] = ((unsigned long long)(unsigned)a << 32) | (unsigned long long)(unsigned)b;
+  kp[i] = on ? (unsigned)pair : 0xffffffffu;
+  idx[i] = (int)i;
+  if (on) atomicAdd(cnt, 1u);
+}
+
+__global__ void k_ts_gather_u32(const unsigned* __restrict__ src, const int* __restrict__ idx, int64_t n,
+                                unsigned* __restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ void k_ts_nfan(const double* __restrict__ poly, const int* __restrict__ order, int n,
+                          unsigned* __restrict__ nt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) nt[i] = (unsigned)poly[HCS_PREC * (size_t)order[i]];
+  if (i == n) nt[i] = 0;
+}
+
+__global__ void k_ts_fans(const double* __restrict__ poly, const int64_t* __restrict__ key,
+                          const int* __restrict__ order, int n, const unsigned* __restrict__ off,
+                          double* __restrict__ tris, int* __restrict__ surf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* P = poly + HCS_PREC * (size_t)order[i];
+  const int nv = (int)P[0];
+  for (int k = 0; k < nv; ++k) {
+    const int k2 = k + 1 < nv ? k + 1 : 0;
+    double* T = tris + HCS_TS_TRI * (size_t)(off[i] + k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) T[c] = P[1 + 3 * k + c], T[3 + c] = P[1 + 3 * k2 + c], T[6 + c] = P[33 + c];
+    T[9] = P[25 + k], T[10] = P[25 + k2], T[11] = P[36];
+    surf[off[i] + k] = (int)key[3 * (size_t)order[i]];
+  }
+}
+
+__device__ inline double ts_len(const double* a, const double* b) {
+  const double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+  return sqrt(dot3(d, d));
+}
+
+/* the reference's loop counts of one triangle */
+__device__ inline void ts_steps(const double* T, double res, int& st, int& st2) {
+  const int st0 = (int)(ts_len(T + 3, T) / res) + 1;
+  const int st1 = (int)(ts_len(T + 6, T) / res) + 1;
+  st2 = (int)(ts_len(T + 6, T) / res) + 1;
+  st = st0 > st1 ? st0 : st1;
+}
+
+__device__ inline void ts_sample(const double* T, double b0, double b1, double b2, double* p, double* e) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) p[c] = (b0 * T[c] + b1 * T[3 + c]) + b2 * T[6 + c];
+  *e = (b0 * T[9] + b1 * T[10]) + b2 * T[11];
+}
+
+__global__ void k_ts_count(const double* __restrict__ tris, int64_t nt, double res, unsigned* __restrict__ cnt) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t > nt) return;
+  if (t == nt) {
+    cnt[t] = 0;
+    return;
+  }
+  int st, st2;
+  ts_steps(tris + HCS_TS_TRI * t, res, st, st2);
+  unsigned na = 0, nb = 0;
+  for (double a = 0; a <= 1; a += 1. / st) ++na;
+  for (double b = 0; b <= 1; b += 1. / st2) ++nb;
+  cnt[t] = na * nb;
+}
+
+__global__ void k_ts_gen(const double* __restrict__ tris, int64_t nt, double res, const unsigned* __restrict__ off,
+                         double* __restrict__ P, double* __restrict__ E) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const double* T = tris + HCS_TS_TRI * t;
+  int st, st2;
+  ts_steps(T, res, st, st2);
+  size_t j = off[t];
+  for (double a = 0; a <= 1; a += 1. / st)
+    for (double b = 0; b <= 1; b += 1. / st2) {
+      ts_sample(T, a, (1 - a) * (1 - b), (1 - a) * b, P + 3 * j, E + j);
+      ++j;
+    }
+}
+
+/* libstdc++ default_random_engine (minstd_rand0, seed 1) through
+ * generate_canonical<double, 53>: two draws, sum = (g1 - 1) + (g2 - 1) r,
+ * r = 2^31 - 2, over r^2; a result >= 1 becomes the largest double below 1.
+ * The long double products of the library are exact here, so double
+ * arithmetic gives the same bits. */
+struct TsRng {
+  unsigned long long x = 1;
+  __device__ double next() {
+    const double r = 2147483646.0;
+    double sum = 0.0, tmp = 1.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      x = (x * 16807ull) % 2147483647ull;
+      sum += (double)(x - 1) * tmp;
+      tmp = tmp * r;
+    }
+    const double u = sum / tmp;
+    return u >= 1.0 ? 0x1.fffffffffffffp-1 : u;
+  }
+};
+
+/* AREA_IMPORTANCE, one thread: write = false counts the samples into *m */
+__global__ void k_ts_area(const double* __restrict__ tris, const int* __restrict__ surf, int64_t nt, double res,
+                          bool write, double* __restrict__ P, double* __restrict__ E,
+                          unsigned long long* __restrict__ m) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  TsRng gen;
+  unsigned long long j = 0;
+  for (int64_t t0 = 0; t0 < nt;) {
+    int64_t t1 = t0;
+    double total = 0.0;
+    while (t1 < nt && surf[t1] == surf[t0]) {
+      const double* T = tris + HCS_TS_TRI * t1;
+      double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]}, e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]}, cr[3];
+      cross3(e1, e2, cr);
+      total += 0.5 * sqrt(dot3(cr, cr));
+      ++t1;
+    }
+    const double area_resolution = res * total;
+    double acc = 0.0, at = 0.0;
+    for (int64_t t = t0; t < t1; ++t) {
+      const double* T = tris + HCS_TS_TRI * t;
+      double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]}, e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]}, cr[3];
+      cross3(e1, e2, cr);
+      at += 0.5 * sqrt(dot3(cr, cr));
+      while (acc < at) {
+        acc += area_resolution;
+        const double u0 = gen.next(), u1 = gen.next();
+        const double a = 1.0 - sqrt(u0);
+        const double b = (1.0 - a) * u1;
+        if (write) ts_sample(T, a, (1 - a) * (1 - b), (1 - a) * b, P + 3 * j, E + j);
+        ++j;
+      }
+    }
+    t0 = t1;
+  }
+  if (!write) *m = j;
+}
+
+struct TsTaxelArgs {
+  double X[12];
+  double margin, res;
+  int method, visualize;
+};
+
+__global__ __launch_bounds__(64) void k_ts_taxel(const double* __restrict__ P, const double* __restrict__ E,
+                                                 int64_t m, TsTaxelArgs a, const double* __restrict__ taxels,
+                                                 double* __restrict__ values) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  double tw[3];
+  xform(a.X, taxels + 3 * (size_t)i, tw);
+  const double tt = dot3(tw, tw), msq = a.margin * a.margin;
+  if (a.method == 0) {
+    double bd = __builtin_inf();
+    long long bj = 0x7fffffffffffffffll;
+    for (int64_t j = lane; j < m; j += 64) {
+      const double* p = P + 3 * j;
+      const double d = ((-2.0 * dot3(tw, p)) + tt) + dot3(p, p);
+      if (d < bd) bd = d, bj = j;
+    }
+    /* first nearest sample: (distance, index) lexicographic minimum */
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double od = __shfl_xor(bd, o, 64);
+      const long long oj = __shfl_xor(bj, o, 64);
+      if (od < bd || (od == bd && oj < bj)) bd = od, bj = oj;
+    }
+    if (lane == 0 && bd < msq) values[i] = (a.visualize && fabs(E[bj]) > 1e-6) ? E[bj] : 0.0;
+  } else {
+    double sum = 0.0;
+    long long ws = 0;
+    for (int64_t j = lane; j < m; j += 64) {
+      const double* p = P + 3 * j;
+      const double d = ((-2.0 * dot3(tw, p)) + tt) + dot3(p, p);
+      if (d < msq) {
+        const double w0 = fmax(0.0, a.margin - sqrt(d));
+        sum += (w0 * w0) * fabs(E[j]);
+        ++ws;
+      }
+    }
+    sum = wave_sum<double>(sum);
+    ws = wave_sum_i(ws);
+    if (lane == 0 && ws > 0) values[i] = sum * a.res;
+  }
+}
+
+}  // namespace hcs
+
+extern "C" {
+
+int hcs_taxel_sensor_update(hcs_ctx* h, const hcs_taxel_sensor* S, double* values, int64_t* nsamples) {
+  if (!h || !S || !values || S->ntaxels < 0 || (S->ntaxels > 0 && !S->taxels) || S->method < 0 || S->method > 3 ||
+      S->sample_method < 0 || S->sample_method > 1 || !(S->sample_resolution > 0) || !(S->include_margin >= 0))
+    return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!c->last_valid)
+    return set_err(c, HCS_ERR_STATE, "hcs_taxel_sensor_update needs a contact step run with keep_polygons = 1");
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  const int npairs = (int)c->last_pairs.size();
+  const int64_t np = c->last_npoly;
+  std::vector<unsigned char> mask(std::max(1, npairs), 0);
+  for (int e = 0; e < npairs; ++e)
+    mask[e] = (c->last_geoms[2 * e] == S->geom || c->last_geoms[2 * e + 1] == S->geom) ? 1 : 0;
+  TaxelSensorBufs& B = c->ts;
+  const size_t n1 = (size_t)std::max<int64_t>(1, np);
+  if ((rc = ensure(c, B.mask, mask.size())) || (rc = ensure(c, B.k64, 2 * sizeof(unsigned long long) * n1)) ||
+      (rc = ensure(c, B.kp, 2 * sizeof(unsigned) * n1)) || (rc = ensure(c, B.idx, 2 * sizeof(int) * n1)) ||
+      (rc = ensure(c, B.cnt, 64)) || (rc = ensure(c, B.nfan, sizeof(unsigned) * (n1 + 1))) ||
+      (rc = ensure(c, B.fan_off, sizeof(unsigned) * (n1 + 1))))
+    return rc;
+  rc = hip_check(c, hipMemcpyAsync(B.mask.ptr, mask.data(), mask.size(), hipMemcpyHostToDevice, s), "upload mask");
+  if (!rc) rc = hip_check(c, hipMemsetAsync(B.cnt.ptr, 0, 64, s), "memset");
+  if (rc) return rc;
+  unsigned long long* k64 = (unsigned long long*)B.k64.ptr;
+  unsigned* kp = (unsigned*)B.kp.ptr;
+  int* idx = (int*)B.idx.ptr;
+  unsigned nsel = 0;
+  if (np > 0) {
+    hipLaunchKernelGGL(k_ts_keys, dim3((np + 255) / 256), dim3(256), 0, s, (const int64_t*)c->d_poly_key.ptr, np,
+                       (const unsigned char*)B.mask.ptr, k64, kp, idx, (unsigned*)B.cnt.ptr);
+    /* stable LSD passes: (element of A, element of B), then pair */
+    size_t b1 = 0, b2 = 0;
+    hipcub::DeviceRadixSort::SortPairs(nullptr, b1, k64, k64 + n1, idx, idx + n1, (int)np, 0, 64, s);
+    hipcub::DeviceRadixSort::SortPairs(nullptr, b2, kp + n1, kp, idx + n1, idx, (int)np, 0, 32, s);
+    if ((rc = ensure(c, B.tmp, std::max<size_t>(8, std::max(b1, b2))))) return rc;
+    b1 = B.tmp.bytes;
+    hipcub::DeviceRadixSort::SortPairs(B.tmp.ptr, b1, k64, k64 + n1, idx, idx + n1, (int)np, 0, 64, s);
+    hipLaunchKernelGGL(k_ts_gather_u32, dim3((np + 255) / 256), dim3(256), 0, s, kp, idx + n1, np, kp + n1);
+    b2 = B.tmp.bytes;
+    hipcub::DeviceRadixSort::SortPairs(B.tmp.ptr, b2, kp + n1, kp, idx + n1, idx, (int)np, 0, 32, s);
+    rc = hip_check(c, hipGetLastError(), "select polygons");
+    if (!rc) rc = hip_check(c, hipMemcpyAsync(&nsel, B.cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, s), "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+  }
+  /* fan triangles of the selected polygons, in order */
+  unsigned ntri = 0;
+  if (nsel > 0) {
+    hipLaunchKernelGGL(k_ts_nfan, dim3((nsel + 1 + 255) / 256), dim3(256), 0, s, (const double*)c->d_poly_rec.ptr,
+                       idx, (int)nsel, (unsigned*)B.nfan.ptr);
+    size_t sb = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, sb, (unsigned*)B.nfan.ptr, (unsigned*)B.fan_off.ptr, (int)nsel + 1, s);
+    if ((rc = ensure(c, B.tmp, std::max<size_t>(sb, 8)))) return rc;
+    sb = B.tmp.bytes;
+    hipcub::DeviceScan::ExclusiveSum(B.tmp.ptr, sb, (unsigned*)B.nfan.ptr, (unsigned*)B.fan_off.ptr, (int)nsel + 1, s);
+    rc = hip_check(c, hipMemcpyAsync(&ntri, (unsigned*)B.fan_off.ptr + nsel, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                     s), "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+  }
+  const size_t nt1 = std::max<size_t>(1, ntri);
+  if ((rc = ensure(c, B.tris, sizeof(double) * HCS_TS_TRI * nt1)) || (rc = ensure(c, B.surf, sizeof(int) * nt1)) ||
+      (rc = ensure(c, B.scnt, sizeof(unsigned) * (nt1 + 1))) || (rc = ensure(c, B.soff, sizeof(unsigned) * (nt1 + 1))))
+    return rc;
+  if (ntri > 0)
+    hipLaunchKernelGGL(k_ts_fans, dim3((nsel + 255) / 256), dim3(256), 0, s, (const double*)c->d_poly_rec.ptr,
+                       (const int64_t*)c->d_poly_key.ptr, idx, (int)nsel, (const unsigned*)B.fan_off.ptr,
+                       (double*)B.tris.ptr, (int*)B.surf.ptr);
+  /* samples */
+  unsigned long long m = 0;
+  if (ntri > 0 && S->sample_method == 0) {
+    hipLaunchKernelGGL(k_ts_count, dim3((ntri + 1 + 255) / 256), dim3(256), 0, s, (const double*)B.tris.ptr,
+                       (int64_t)ntri, S->sample_resolution, (unsigned*)B.scnt.ptr);
+    size_t sb = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, sb, (unsigned*)B.scnt.ptr, (unsigned*)B.soff.ptr, (int)ntri + 1, s);
+    if ((rc = ensure(c, B.tmp, std::max<size_t>(sb, 8)))) return rc;
+    sb = B.tmp.bytes;
+    hipcub::DeviceScan::ExclusiveSum(B.tmp.ptr, sb, (unsigned*)B.scnt.ptr, (unsigned*)B.soff.ptr, (int)ntri + 1, s);
+    unsigned m32 = 0;
+    rc = hip_check(c, hipMemcpyAsync(&m32, (unsigned*)B.soff.ptr + ntri, sizeof(unsigned), hipMemcpyDeviceToHost, s),
+                   "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+    m = m32;
+  } else if (ntri > 0) {
+    hipLaunchKernelGGL(k_ts_area, dim3(1), dim3(64), 0, s, (const double*)B.tris.ptr, (const int*)B.surf.ptr,
+                       (int64_t)ntri, S->sample_resolution, false, nullptr, nullptr,
+                       (unsigned long long*)B.cnt.ptr);
+    rc = hip_check(c, hipMemcpyAsync(&m, B.cnt.ptr, sizeof(m), hipMemcpyDeviceToHost, s), "dl");
+    if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+    if (rc) return rc;
+  }
+  if (nsamples) *nsamples = (int64_t)m;
+  if (m == 0) {
+    for (int i = 0; i < S->ntaxels; ++i) values[i] = 0.0;
+    return HCS_OK;
+  }
+  if ((rc = ensure(c, B.P, sizeof(double) * 3 * m)) || (rc = ensure(c, B.E, sizeof(double) * m)) ||
+      (rc = ensure(c, B.taxel, sizeof(double) * 3 * std::max(1, S->ntaxels))) ||
+      (rc = ensure(c, B.val, sizeof(double) * std::max(1, S->ntaxels))))
+    return rc;
+  if (S->sample_method == 0)
+    hipLaunchKernelGGL(k_ts_gen, dim3((ntri + 255) / 256), dim3(256), 0, s, (const double*)B.tris.ptr, (int64_t)ntri,
+                       S->sample_resolution, (const unsigned*)B.soff.ptr, (double*)B.P.ptr, (double*)B.E.ptr);
+  else
+    hipLaunchKernelGGL(k_ts_area, dim3(1), dim3(64), 0, s, (const double*)B.tris.ptr, (const int*)B.surf.ptr,
+                       (int64_t)ntri, S->sample_resolution, true, (double*)B.P.ptr, (double*)B.E.ptr,
+                       (unsigned long long*)B.cnt.ptr);
+  if (S->ntaxels == 0) return hip_check(c, hipGetLastError(), "samples");
+  TsTaxelArgs a;
+  std::memcpy(a.X, S->X_W_sensor, sizeof a.X);
+  a.margin = S->include_margin, a.res = S->sample_resolution, a.method = S->method, a.visualize = S->visualize != 0;
+  rc = hip_check(c, hipMemcpyAsync(B.taxel.ptr, S->taxels, sizeof(double) * 3 * S->ntaxels, hipMemcpyHostToDevice, s),
+                 "upload taxels");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(B.val.ptr, values, sizeof(double) * S->ntaxels, hipMemcpyHostToDevice, s),
+                   "upload values");
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ts_taxel, dim3(S->ntaxels), dim3(64), 0, s, (const double*)B.P.ptr, (const double*)B.E.ptr,
+                     (int64_t)m, a, (const double*)B.taxel.ptr, (double*)B.val.ptr);
+  rc = hip_check(c, hipGetLastError(), "taxel sensor");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(values, B.val.ptr, sizeof(double) * S->ntaxels, hipMemcpyDeviceToHost, s), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+  return rc;
+}
+
+}  // extern "C"

@@ -209,6 +209,27 @@ class Engine:
             o += cx * cy
         return res
 
+    def taxel_sensor(self, geom, X_W_sensor, taxels, include_margin, sample_resolution, method="squared",
+                     sample_method="default", visualize=False, values=None):
+        """TaxelSensor::internal_update (S/src/taxel_sensor.cpp:158-478) on
+        the polygons of the preceding contact_step(keep_polygons=True).
+        Returns (values, number of samples); values no sample reaches keep
+        their previous value (default 0)."""
+        tx = A.as_f64(taxels, (-1, 3))
+        vals = np.zeros(tx.shape[0]) if values is None else A.as_f64(values).copy()
+        s = A.TaxelSensor()
+        s.geom, s.ntaxels = int(geom), tx.shape[0]
+        s.method = {"closest": 0, "weighted": 1, "mean": 2, "squared": 3}[method]
+        s.sample_method = {"default": 0, "area_importance": 1}[sample_method]
+        s.visualize = int(bool(visualize))
+        s.include_margin, s.sample_resolution = float(include_margin), float(sample_resolution)
+        for i, v in enumerate(A.as_f64(X_W_sensor)):
+            s.X_W_sensor[i] = v
+        s.taxels = A.dptr(tx)
+        n = C.c_int64()
+        self._chk(self.lib.hcs_taxel_sensor_update(self.h, C.byref(s), A.dptr(vals), C.byref(n)))
+        return vals, n.value
+
     # ---- differentiation
     def jacobian(self, pairs, representation=KTRIANGLE, apply_forces=True, dt=0.001):
         n = len(pairs)

@@ -939,6 +939,144 @@ int orc_taxels(const double* polys, int64_t npoly, const double* sensor, int S, 
   return 0;
 }
 
+/* std::default_random_engine of libstdc++ (minstd_rand0: x <- 16807 x mod
+ * 2^31 - 1, default seed 1) feeding std::uniform_real_distribution<double>
+ * (0, 1), i.e. generate_canonical<double, 53>: two draws (k = max(1,
+ * (53 + 30 - 1) / 30)), sum = (g1 - 1) + (g2 - 1) r with r = 2^31 - 2,
+ * divided by r^2 (r^2 formed in long double, then rounded to double);
+ * a result >= 1 becomes the largest double below 1. */
+struct MinstdUniform {
+  uint64_t x = 1;
+  uint64_t next() {
+    x = (x * 16807u) % 2147483647u;
+    return x;
+  }
+  double operator()() {
+    const long double r = 2147483646.0L;
+    double sum = 0.0, tmp = 1.0;
+    for (int k = 0; k < 2; ++k) {
+      sum += (double)(next() - 1) * tmp;
+      tmp = (double)(tmp * r);
+    }
+    double u = sum / tmp;
+    if (u >= 1.0) u = std::nextafter(1.0, 0.0);
+    return u;
+  }
+};
+
+/* TaxelSensor::internal_update (S/src/taxel_sensor.cpp:158-478), FP64.
+ * tris: kTriangle fan triangles of the contact surfaces touching the
+ * sensor, in order, each [v0, v1, v2 (W), e0, e1, e2] (12 doubles), with
+ * surf[t] the index of the surface the triangle belongs to (non-decreasing).
+ * Samples: DEFAULT (:174-203, st1 and st2 both from |v2 - v0|) or
+ * AREA_IMPORTANCE (:205-246, one generator for the whole update, area
+ * accumulators per surface).  Taxel positions: X_W_sensor applied to the
+ * taxels (sensor frame).  dist = (-2 t.p + |t|^2) + |p|^2.  Methods:
+ * CLOSEST (:285-319: the first nearest sample; within the margin the value
+ * is its pressure only when visualize is set and |p| > 1e-6, else 0);
+ * WEIGHTED and MEAN fall through to SQUARED (:320-457, no break), so all
+ * three give SQUARED: sum over samples within the margin of
+ * max(0, margin - sqrt(dist))^2 |p|, times sample_resolution.  Taxels no
+ * sample reaches keep their value; with no samples at all every taxel is
+ * 0 (:458-477).  Returns the number of samples. */
+int64_t orc_taxel_sensor(const double* tris, const int32_t* surf, int64_t ntri, const double* X, const double* taxels,
+                         int ntaxels, double margin, double res, int method, int sample_method, int visualize,
+                         double* values) {
+  std::vector<double> P, E;
+  auto add = [&](const double* T, double b0, double b1, double b2) {
+    for (int c = 0; c < 3; ++c) P.push_back((b0 * T[c] + b1 * T[3 + c]) + b2 * T[6 + c]);
+    E.push_back((b0 * T[9] + b1 * T[10]) + b2 * T[11]);
+  };
+  auto dist3 = [](const double* a, const double* b) {
+    const double d[3] = {a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+    return std::sqrt(dot3(d, d));
+  };
+  if (sample_method == 0) {
+    for (int64_t t = 0; t < ntri; ++t) {
+      const double* T = tris + 12 * t;
+      const int st0 = (int)(dist3(T + 3, T) / res) + 1;
+      const int st1 = (int)(dist3(T + 6, T) / res) + 1;
+      const int st2 = (int)(dist3(T + 6, T) / res) + 1;
+      const int st = std::max(st0, st1);
+      for (double a = 0; a <= 1; a += 1. / st)
+        for (double b = 0; b <= 1; b += 1. / st2) add(T, a, (1 - a) * (1 - b), (1 - a) * b);
+    }
+  } else {
+    MinstdUniform gen;
+    for (int64_t t0 = 0; t0 < ntri;) {
+      int64_t t1 = t0;
+      double total = 0.0;
+      std::vector<double> area;
+      while (t1 < ntri && surf[t1] == surf[t0]) {
+        const double* T = tris + 12 * t1;
+        double e1[3], e2[3], cr[3];
+        sub3(T + 3, T, e1);
+        sub3(T + 6, T, e2);
+        cross3(e1, e2, cr);
+        area.push_back(0.5 * std::sqrt(dot3(cr, cr)));
+        total += area.back();
+        ++t1;
+      }
+      const double area_resolution = res * total;
+      double acc = 0.0, at = 0.0;
+      for (int64_t t = t0; t < t1; ++t) {
+        at += area[t - t0];
+        while (acc < at) {
+          acc += area_resolution;
+          const double u0 = gen(), u1 = gen();
+          const double a = 1.0 - std::sqrt(u0);
+          const double b = (1.0 - a) * u1;
+          add(tris + 12 * t, a, (1 - a) * (1 - b), (1 - a) * b);
+        }
+      }
+      t0 = t1;
+    }
+  }
+  const int64_t m = (int64_t)E.size();
+  if (m == 0) {
+    for (int i = 0; i < ntaxels; ++i) values[i] = 0.0;
+    return 0;
+  }
+  const double msq = margin * margin;
+  for (int i = 0; i < ntaxels; ++i) {
+    double tw[3];
+    xform(X, taxels + 3 * i, tw);
+    const double tt = dot3(tw, tw);
+    auto dist = [&](int64_t j) {
+      const double* p = &P[3 * j];
+      return ((-2.0 * dot3(tw, p)) + tt) + dot3(p, p);
+    };
+    if (method == 0) {
+      int64_t jm = 0;
+      double dm = dist(0);
+      for (int64_t j = 1; j < m; ++j) {
+        const double d = dist(j);
+        if (d < dm) dm = d, jm = j;
+      }
+      if (dm < msq) values[i] = (visualize && std::fabs(E[jm]) > 1e-6) ? E[jm] : 0.0;
+    } else {
+      double sum = 0.0;
+      int64_t ws = 0;
+      for (int64_t j = 0; j < m; ++j) {
+        const double d = dist(j);
+        if (d < msq) {
+          const double w0 = std::max(0.0, margin - std::sqrt(d));
+          sum += (w0 * w0) * std::fabs(E[j]);
+          ++ws;
+        }
+      }
+      if (ws > 0) values[i] = sum * res;
+    }
+  }
+  return m;
+}
+
+/* The first n values of MinstdUniform (for pinning it against libstdc++). */
+void orc_minstd_uniform(int n, double* out) {
+  MinstdUniform g;
+  for (int i = 0; i < n; ++i) out[i] = g();
+}
+
 /* collision_cb SOFT-SOFT branch (plugin.cpp:280-286): Drake semantics of
  * ComputeContactSurfaceFromCompliantVolumes -- per candidate tet pair the
  * equal-pressure plane f_A = f_B (normal along grad f_A - grad f_B, into A),

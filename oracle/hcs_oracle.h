@@ -106,6 +106,15 @@ int orc_soft_rigid_step(void* prep, const double* VR, int nvr, const int32_t* F,
 int orc_taxels(const double* polys, int64_t npoly, const double* sensor, int S, int window,
                double sigma, int cx, int cy, double* out);
 
+/* TaxelSensor update over kTriangle fan triangles (12 doubles each: 3
+ * vertices in W, 3 vertex pressures) with their surface index; method 0
+ * closest, 1 weighted, 2 mean, 3 squared; sample_method 0 default, 1 area
+ * importance; values in/out.  Returns the number of samples. */
+int64_t orc_taxel_sensor(const double* tris, const int32_t* surf, int64_t ntri, const double* X_W_sensor,
+                         const double* taxels, int ntaxels, double margin, double sample_resolution, int method,
+                         int sample_method, int visualize, double* values);
+void orc_minstd_uniform(int n, double* out);
+
 /* Soft-soft contact step (tet-tet equal-pressure planes).
  *  pose = [X_WA(12), X_WB(12), V_WA(6), V_WB(6)]
  *  params = [E_A, d_A, mu_A, E_B, d_B, mu_B, dt, apply, repr, A is M]

@@ -51,6 +51,10 @@ def lib():
                                     _D, _I64, _D, _I64, C.c_int64, _D, _I64, C.c_int64]
         L.orc_soft_plane.argtypes = [_D, C.c_int, _I32, C.c_int, _D, _D, _D, _D, _I64, _D, _I64, C.c_int64, _D,
                                      _I64, C.c_int64]
+        L.orc_taxel_sensor.restype = C.c_int64
+        L.orc_taxel_sensor.argtypes = [_D, _I32, C.c_int64, _D, _D, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, _D]
+        L.orc_minstd_uniform.argtypes = [C.c_int, _D]
         L.orc_soft_prepare.restype = C.c_void_p
         L.orc_soft_prepare.argtypes = [_D, C.c_int, _I32, C.c_int, _D]
         L.orc_soft_free.argtypes = [C.c_void_p]
@@ -235,3 +239,42 @@ class PreparedSoft:
 
     def __del__(self):
         self.close()
+
+
+METHODS = {"closest": 0, "weighted": 1, "mean": 2, "squared": 3}
+SAMPLE_METHODS = {"default": 0, "area_importance": 1}
+
+
+def fan_triangles(polys, surface_ids):
+    """kTriangle fan (v_k, v_k+1, C) of polygon records in order: (ntri, 12)
+    rows [v0, v1, v2, e0, e1, e2] and the surface index of each."""
+    rows, surf = [], []
+    for P, sid in zip(np.asarray(polys), surface_ids):
+        n = int(P[0])
+        V = P[1:25].reshape(8, 3)
+        e = P[25:33]
+        C_, eC = P[33:36], P[36]
+        for k in range(n):
+            k2 = (k + 1) % n
+            rows.append(np.concatenate([V[k], V[k2], C_, [e[k], e[k2], eC]]))
+            surf.append(sid)
+    return np.array(rows, float).reshape(-1, 12), np.array(surf, np.int32)
+
+
+def taxel_sensor(tris, surf, X_W_sensor, taxels, include_margin, sample_resolution, method="squared",
+                 sample_method="default", visualize=False, values=None):
+    """TaxelSensor::internal_update (S/src/taxel_sensor.cpp:158-478) over
+    fan triangles; returns (values, number of samples)."""
+    tris, X, tx = _d(tris).reshape(-1, 12), _d(X_W_sensor), _d(taxels).reshape(-1, 3)
+    surf = _i(surf)
+    vals = np.zeros(tx.shape[0]) if values is None else _d(values).copy()
+    m = lib().orc_taxel_sensor(dp(tris), ip(surf), tris.shape[0], dp(X), dp(tx), tx.shape[0], float(include_margin),
+                               float(sample_resolution), METHODS[method], SAMPLE_METHODS[sample_method],
+                               int(visualize), dp(vals))
+    return vals, int(m)
+
+
+def minstd_uniform(n):
+    out = np.zeros(n)
+    lib().orc_minstd_uniform(int(n), dp(out))
+    return out

@@ -507,3 +507,42 @@ def test_batched_soft_soft_matches_plain_and_oracle(engine):
     for i in range(len(scs)):
         assert_wrench_close(W2[i], W[i], rtol=1e-12)
     b.close()
+
+
+# -------------------------------------------------------------- taxel sensor
+
+
+def _pad_taxels(sc, n=16):
+    """A TaxelSensor-style taxel array on the pad surface (pad frame)."""
+    xs, zs = sc["grid"]["half_size"][0], sc["grid"]["half_size"][2]
+    g = (np.arange(n) + 0.5) / n * 2 * xs - xs
+    return np.array([[x, y, zs] for y in g for x in g])
+
+
+@pytest.mark.parametrize("method,sample_method,visualize", [("squared", "default", False), ("mean", "default", False),
+                                                            ("closest", "default", True),
+                                                            ("closest", "default", False),
+                                                            ("squared", "area_importance", False),
+                                                            ("closest", "area_importance", True)])
+def test_taxel_sensor_matches_oracle(engine, method, sample_method, visualize):
+    say("taxel sensor", method, sample_method, visualize)
+    sc = S.tactile_pad(1, L=3, K=3)
+    ha, hb = add_scene(engine, sc)
+    other = S.sphere_box(2)  # a second pair that does not touch the pad
+    hc, hd = add_scene(engine, other)
+    engine.contact_step([pair_of(sc, ha, hb), pair_of(other, hc, hd)], keep_polygons=True)
+    taxels = _pad_taxels(sc)
+    prev = np.full(taxels.shape[0], -3.0)
+    margin, res = 0.004, 0.001 if sample_method == "default" else 0.002
+    vals, m = engine.taxel_sensor(hb, sc["X_WB"], taxels, margin, res, method, sample_method, visualize, values=prev)
+    ref = oracle_soft_rigid(sc)
+    tris, surf = O.fan_triangles(ref["polys"], np.zeros(len(ref["polys"]), int))
+    rvals, rm = O.taxel_sensor(tris, surf, sc["X_WB"], taxels, margin, res, method, sample_method, visualize,
+                               values=prev)
+    assert m == rm and m > 0
+    reached = rvals != -3.0
+    assert reached.any() and (~reached).any()
+    np.testing.assert_array_equal(vals == -3.0, ~reached)
+    np.testing.assert_allclose(vals, rvals, rtol=1e-10, atol=1e-12 * np.abs(rvals).max())
+    if method == "closest":
+        np.testing.assert_array_equal(vals, rvals)

@@ -497,3 +497,96 @@ def test_half_space_equals_a_plane_grid_mesh_on_force_and_area():
         assert hs["faces"][:, 9].sum() == pytest.approx(gr["faces"][:, 9].sum(), rel=1e-12)
         np.testing.assert_allclose(hs["wrench"][:3], gr["wrench"][:3], rtol=1e-11,
                                    atol=1e-12 * np.abs(gr["wrench"][:3]).max())
+
+
+# -------------------------------------------------------------- taxel sensor
+
+
+def test_area_importance_generator_is_libstdcxx_default_random_engine(tmp_path):
+    """The restated generator against std::default_random_engine feeding
+    std::uniform_real_distribution<double>(0, 1), compiled here."""
+    src = tmp_path / "rng.cpp"
+    src.write_text('#include <cstdio>\n#include <random>\nint main() { std::default_random_engine g; '
+                   'std::uniform_real_distribution<double> u(0.0, 1.0); '
+                   'for (int i = 0; i < 4000; ++i) std::printf("%a\\n", u(g)); }\n')
+    exe = tmp_path / "rng"
+    import subprocess
+    subprocess.run(["g++", "-O0", str(src), "-o", str(exe)], check=True, timeout=120)
+    ref = np.array([float.fromhex(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                               text=True).stdout.split()])
+    np.testing.assert_array_equal(O.minstd_uniform(4000), ref)
+
+
+def _tri_rows(V, e):
+    return np.concatenate([np.asarray(V, float).reshape(9), np.asarray(e, float)])[None, :]
+
+
+def test_default_sample_count_follows_the_reference_loops():
+    V = [[0, 0, 0], [0.013, 0, 0], [0, 0.0071, 0]]
+    res = 0.001
+    st0 = int(np.linalg.norm(np.subtract(V[1], V[0])) / res) + 1
+    st2 = int(np.linalg.norm(np.subtract(V[2], V[0])) / res) + 1
+    st = max(st0, st2)  # st1 is computed from v2 - v0 as well
+
+    def count(step):
+        n, a = 0, 0.0
+        while a <= 1:
+            n += 1
+            a += 1.0 / step
+        return n
+
+    _, m = O.taxel_sensor(_tri_rows(V, [1, 1, 1]), [0], pose(), np.zeros((1, 3)), 0.01, res)
+    assert m == count(st) * count(st2)
+
+
+def test_closest_reads_the_vertex_pressure_only_when_visualizing():
+    V = [[0, 0, 0], [0.01, 0, 0], [0, 0.01, 0]]
+    tri = _tri_rows(V, [100.0, 200.0, 300.0])
+    taxels = np.array([[0.01, 0, 0], [0, 0.01, 0], [0, 0, 0], [0.5, 0.5, 0.5]])
+    prev = np.full(4, 7.0)
+    vals, _ = O.taxel_sensor(tri, [0], pose(), taxels, 0.002, 0.001, "closest", visualize=True, values=prev)
+    # a = b = 0 is always a sample, at v1; the loops a, b += 1/st stop short
+    # of 1 here (11 steps of 1/11 exceed 1), so v0 and v2 are not sampled and
+    # their taxels read the nearest interior sample
+    assert vals[0] == 200.0
+    assert 200.0 < vals[1] < 300.0 and 100.0 < vals[2] < 200.0
+    assert vals[3] == 7.0  # out of the margin: the previous value stays
+    vals, _ = O.taxel_sensor(tri, [0], pose(), taxels, 0.002, 0.001, "closest", visualize=False, values=prev)
+    np.testing.assert_array_equal(vals, [0.0, 0.0, 0.0, 7.0])
+
+
+def test_weighted_and_mean_fall_through_to_squared():
+    rng = np.random.default_rng(3)
+    V = rng.uniform(-0.01, 0.01, (3, 3))
+    tri = _tri_rows(V, [50.0, 80.0, 120.0])
+    taxels = rng.uniform(-0.01, 0.01, (20, 3))
+    out = [O.taxel_sensor(tri, [0], pose(), taxels, 0.004, 0.0005, m, values=np.full(20, -1.0))[0]
+           for m in ("weighted", "mean", "squared")]
+    np.testing.assert_array_equal(out[0], out[2])
+    np.testing.assert_array_equal(out[1], out[2])
+    assert (out[2] > 0).any() and (out[2] == -1.0).any()
+
+
+def test_squared_is_linear_in_pressure_and_rigidly_invariant():
+    rng = np.random.default_rng(5)
+    V = rng.uniform(-0.01, 0.01, (3, 3))
+    taxels = rng.uniform(-0.008, 0.008, (16, 3))
+    e = np.array([10.0, 40.0, 90.0])
+    a, _ = O.taxel_sensor(_tri_rows(V, e), [0], pose(), taxels, 0.005, 0.0007)
+    b, _ = O.taxel_sensor(_tri_rows(V, 2 * e), [0], pose(), taxels, 0.005, 0.0007)
+    np.testing.assert_array_equal(b, 2 * a)  # doubling every pressure is exact
+    # move the surface and the sensor together: same values up to rounding
+    R, t = rot([0.3, 0.5, -0.2], 0.7), np.array([0.02, -0.01, 0.03])
+    VW = np.asarray(V) @ R.T + t
+    c, _ = O.taxel_sensor(_tri_rows(VW, e), [0], pose(R, t), taxels, 0.005, 0.0007)
+    np.testing.assert_allclose(c, a, rtol=1e-9, atol=1e-12 * np.abs(a).max())
+
+
+def test_area_importance_samples_about_one_per_resolution_step():
+    rng = np.random.default_rng(7)
+    tris = np.concatenate([_tri_rows(rng.uniform(-0.01, 0.01, (3, 3)), [1, 2, 3]) for _ in range(6)])
+    surf = [0, 0, 0, 1, 1, 1]
+    res = 0.05  # area_resolution = 5% of each surface's area
+    _, m = O.taxel_sensor(tris, surf, pose(), np.zeros((1, 3)), 0.01, res, sample_method="area_importance")
+    # per surface the accumulator steps through [0, total) in res * total steps
+    assert 2 * 20 <= m <= 2 * 21
