@@ -133,6 +133,25 @@ typedef struct hcs_taxel_sensor {
   const double* taxels;  /* ntaxels x 3, host memory */
 } hcs_taxel_sensor;
 
+/* CurvedSensor (S/src/curved_sensor.cpp:28-380): the surface samples of the
+ * sensor geometry (points and normals in its frame) and, per taxel, the
+ * samples it averages with their weights (CSR: taxel_ptr[ntaxels + 1],
+ * sample_idx / weights[taxel_ptr[ntaxels]]), as the sensor's load step
+ * builds them. */
+typedef struct hcs_curved_sensor {
+  int32_t geom;             /* mesh handle of the sensor geometry */
+  int32_t ntaxels;
+  int32_t nsamples;
+  int32_t reserved;
+  double include_margin;
+  double X_W_sensor[12];    /* sensor geom pose */
+  const double* points;     /* nsamples x 3, host memory */
+  const double* normals;    /* nsamples x 3 */
+  const int32_t* taxel_ptr; /* ntaxels + 1 */
+  const int32_t* sample_idx;
+  const double* weights;
+} hcs_curved_sensor;
+
 /* ---- lifetime (plugin constructor/load/destructor, plugin.cpp:191-245) */
 int hcs_create(hcs_ctx** ctx, int device_ordinal);
 /* Also destroys the context's remaining batches; their handles become
@@ -189,6 +208,10 @@ int hcs_taxels(hcs_ctx* ctx, int32_t nsensors, const hcs_taxel_grid* g, double* 
  * values (ntaxels, in/out): taxels no sample reaches keep their value, as
  * the reference's message does; *nsamples (optional) = surface samples. */
 int hcs_taxel_sensor_update(hcs_ctx* ctx, const hcs_taxel_sensor* sensor, double* values, int64_t* nsamples);
+/* CurvedSensor::internal_update (S/src/curved_sensor.cpp:388-481) on the
+ * polygons of the last hcs_contact_step run with keep_polygons = 1: one ray
+ * per surface sample along its inward normal; values (ntaxels) all written. */
+int hcs_curved_sensor_update(hcs_ctx* ctx, const hcs_curved_sensor* sensor, double* values);
 
 /* ---- batched form (no reference counterpart: the reference runs one pair
  * at a time inside mj_collision).  Thousands of independent pairs, one

@@ -59,6 +59,13 @@ class TaxelSensor(C.Structure):
                 ("taxels", C.POINTER(C.c_double))]
 
 
+class CurvedSensor(C.Structure):
+    _fields_ = [("geom", C.c_int32), ("ntaxels", C.c_int32), ("nsamples", C.c_int32), ("reserved", C.c_int32),
+                ("include_margin", C.c_double), ("X_W_sensor", C.c_double * 12), ("points", C.POINTER(C.c_double)),
+                ("normals", C.POINTER(C.c_double)), ("taxel_ptr", C.POINTER(C.c_int32)),
+                ("sample_idx", C.POINTER(C.c_int32)), ("weights", C.POINTER(C.c_double))]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
@@ -82,6 +89,7 @@ SIGNATURES = {
     "hcs_taxel_dims": (C.c_int, [C.POINTER(TaxelGrid), _I32, _I32]),
     "hcs_taxels": (C.c_int, [_P, C.c_int32, C.POINTER(TaxelGrid), _D]),
     "hcs_taxel_sensor_update": (C.c_int, [_P, C.POINTER(TaxelSensor), _D, _I64]),
+    "hcs_curved_sensor_update": (C.c_int, [_P, C.POINTER(CurvedSensor), _D]),
     "hcs_batch_create": (C.c_int, [_P, C.c_int32, _I32, C.POINTER(StepOpts), C.POINTER(_P)]),
     "hcs_batch_destroy": (C.c_int, [_P]),
     "hcs_batch_set_state": (C.c_int, [_P, _D]),

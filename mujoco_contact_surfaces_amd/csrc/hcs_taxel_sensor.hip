@@ -195,6 +195,11 @@ __global__ void k_ts_area(const double* __restrict__ tris, const int* __restrict
   if (!write) *m = j;
 }
 
+struct CsArgs {
+  double X[12];
+  double margin;
+};
+
 struct TsTaxelArgs {
   double X[12];
   double margin, res;
@@ -242,25 +247,96 @@ __global__ __launch_bounds__(64) void k_ts_taxel(const double* __restrict__ P, c
   }
 }
 
+/* CurvedSensor ray of one surface sample (one thread): the sample moved by
+ * X (normal rotated, not renormalised), ray from p + n 1e-8 along -n,
+ * Moeller-Trumbore (S/src/bvh.cpp:49-74) in FP64 over the fan triangles,
+ * nearest t (first triangle in order on ties); raw pressure when 0 < t <
+ * margin.  Triangles whose box misses the box of the ray segment
+ * [0, margin] (widened by 1e-9 of its largest coordinate) are skipped: they
+ * cannot give an accepted hit, and one with t >= margin never changes
+ * whether the nearest hit is accepted. */
+__global__ __launch_bounds__(256) void k_cs_rays(const double* __restrict__ tris, int64_t nt, CsArgs a,
+                                                 const double* __restrict__ pts, const double* __restrict__ nrms,
+                                                 int m, double* __restrict__ raw, unsigned char* __restrict__ hit) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  double pw[3], nw[3], O[3], D[3];
+  xform(a.X, pts + 3 * (size_t)j, pw);
+  rot(a.X, nrms + 3 * (size_t)j, nw);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) O[c] = pw[c] + nw[c] * 1e-8, D[c] = -nw[c];
+  double slo[3], shi[3], E[3], sc = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) E[c] = O[c] + D[c] * a.margin, sc = fmax(sc, fmax(fabs(O[c]), fabs(E[c])));
+  /* one slack for all axes, on the scale of the largest coordinate: the
+   * rounding of t moves the computed hit along every axis by that much */
+  const double sl = 1e-9 * sc + 1e-300;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) slo[c] = fmin(O[c], E[c]) - sl, shi[c] = fmax(O[c], E[c]) + sl;
+  double best = 1e30, bu = 0.0, bv = 0.0;
+  int64_t bt = -1;
+  for (int64_t t = 0; t < nt; ++t) {
+    const double* T = tris + HCS_TS_TRI * t;
+    bool out = false;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double lo = fmin(fmin(T[c], T[3 + c]), T[6 + c]), hi = fmax(fmax(T[c], T[3 + c]), T[6 + c]);
+      out = out || hi < slo[c] || lo > shi[c];
+    }
+    if (out) continue;
+    const double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]}, e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]};
+    double h[3];
+    cross3(D, e2, h);
+    const double det = dot3(e1, h);
+    if (fabs(det) < 1e-10) continue;
+    const double f = 1.0 / det;
+    const double sv[3] = {O[0] - T[0], O[1] - T[1], O[2] - T[2]};
+    const double u = f * dot3(sv, h);
+    if (u < 0.0 || u > 1.0) continue;
+    double qv[3];
+    cross3(sv, e1, qv);
+    const double v = f * dot3(D, qv);
+    if (v < 0.0 || u + v > 1.0) continue;
+    const double tt = f * dot3(e2, qv);
+    if (tt > 0.0 && tt < best) best = tt, bu = u, bv = v, bt = t;
+  }
+  const bool ok = bt >= 0 && best < a.margin && best > 0.0;
+  double r = 0.0;
+  if (ok) {
+    const double* T = tris + HCS_TS_TRI * bt;
+    const double b0 = 1 - bu - bv;
+    r = (b0 * T[9] + bu * T[10]) + bv * T[11];
+  }
+  raw[j] = r;
+  hit[j] = ok ? 1 : 0;
+}
+
+/* taxel i: sum over its samples of weight * raw, in list order */
+__global__ void k_cs_taxels(const int* __restrict__ ptr, const int* __restrict__ idx, const double* __restrict__ w,
+                            int n, const double* __restrict__ raw, const unsigned char* __restrict__ hit,
+                            double* __restrict__ values) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double pr = 0.0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+    if (hit[idx[k]]) pr += w[k] * raw[idx[k]];
+  values[i] = pr;
+}
+
 }  // namespace hcs
 
 extern "C" {
 
-int hcs_taxel_sensor_update(hcs_ctx* h, const hcs_taxel_sensor* S, double* values, int64_t* nsamples) {
-  if (!h || !S || !values || S->ntaxels < 0 || (S->ntaxels > 0 && !S->taxels) || S->method < 0 || S->method > 3 ||
-      S->sample_method < 0 || S->sample_method > 1 || !(S->sample_resolution > 0) || !(S->include_margin >= 0))
-    return HCS_ERR_ARG;
-  Ctx* c = &h->c;
-  if (!c->last_valid)
-    return set_err(c, HCS_ERR_STATE, "hcs_taxel_sensor_update needs a contact step run with keep_polygons = 1");
-  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
-  if (rc) return rc;
+/* The kTriangle fan triangles of the kept polygons of the pairs touching
+ * geometry geom, in key order, into c->ts.tris / c->ts.surf. */
+static int sensor_fans(Ctx* c, int32_t geom, unsigned& ntri) {
+  int rc;
   hipStream_t s = c->stream;
   const int npairs = (int)c->last_pairs.size();
   const int64_t np = c->last_npoly;
   std::vector<unsigned char> mask(std::max(1, npairs), 0);
   for (int e = 0; e < npairs; ++e)
-    mask[e] = (c->last_geoms[2 * e] == S->geom || c->last_geoms[2 * e + 1] == S->geom) ? 1 : 0;
+    mask[e] = (c->last_geoms[2 * e] == geom || c->last_geoms[2 * e + 1] == geom) ? 1 : 0;
   TaxelSensorBufs& B = c->ts;
   const size_t n1 = (size_t)std::max<int64_t>(1, np);
   if ((rc = ensure(c, B.mask, mask.size())) || (rc = ensure(c, B.k64, 2 * sizeof(unsigned long long) * n1)) ||
@@ -294,7 +370,7 @@ int hcs_taxel_sensor_update(hcs_ctx* h, const hcs_taxel_sensor* S, double* value
     if (rc) return rc;
   }
   /* fan triangles of the selected polygons, in order */
-  unsigned ntri = 0;
+  ntri = 0;
   if (nsel > 0) {
     hipLaunchKernelGGL(k_ts_nfan, dim3((nsel + 1 + 255) / 256), dim3(256), 0, s, (const double*)c->d_poly_rec.ptr,
                        idx, (int)nsel, (unsigned*)B.nfan.ptr);
@@ -316,6 +392,22 @@ int hcs_taxel_sensor_update(hcs_ctx* h, const hcs_taxel_sensor* S, double* value
     hipLaunchKernelGGL(k_ts_fans, dim3((nsel + 255) / 256), dim3(256), 0, s, (const double*)c->d_poly_rec.ptr,
                        (const int64_t*)c->d_poly_key.ptr, idx, (int)nsel, (const unsigned*)B.fan_off.ptr,
                        (double*)B.tris.ptr, (int*)B.surf.ptr);
+  return hip_check(c, hipGetLastError(), "sensor triangles");
+}
+
+int hcs_taxel_sensor_update(hcs_ctx* h, const hcs_taxel_sensor* S, double* values, int64_t* nsamples) {
+  if (!h || !S || !values || S->ntaxels < 0 || (S->ntaxels > 0 && !S->taxels) || S->method < 0 || S->method > 3 ||
+      S->sample_method < 0 || S->sample_method > 1 || !(S->sample_resolution > 0) || !(S->include_margin >= 0))
+    return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (!c->last_valid)
+    return set_err(c, HCS_ERR_STATE, "hcs_taxel_sensor_update needs a contact step run with keep_polygons = 1");
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  TaxelSensorBufs& B = c->ts;
+  unsigned ntri = 0;
+  if ((rc = sensor_fans(c, S->geom, ntri))) return rc;
   /* samples */
   unsigned long long m = 0;
   if (ntri > 0 && S->sample_method == 0) {
@@ -371,6 +463,67 @@ int hcs_taxel_sensor_update(hcs_ctx* h, const hcs_taxel_sensor* S, double* value
   rc = hip_check(c, hipGetLastError(), "taxel sensor");
   if (!rc)
     rc = hip_check(c, hipMemcpyAsync(values, B.val.ptr, sizeof(double) * S->ntaxels, hipMemcpyDeviceToHost, s), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
+  return rc;
+}
+
+int hcs_curved_sensor_update(hcs_ctx* h, const hcs_curved_sensor* S, double* values) {
+  if (!h || !S || !values || S->ntaxels < 0 || S->nsamples < 0 || !(S->include_margin >= 0) ||
+      (S->ntaxels > 0 && !S->taxel_ptr) || (S->nsamples > 0 && (!S->points || !S->normals)))
+    return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (S->ntaxels == 0) return HCS_OK;
+  const int nnz = S->taxel_ptr[S->ntaxels];
+  if (S->taxel_ptr[0] != 0 || nnz < 0 || (nnz > 0 && (!S->sample_idx || !S->weights)))
+    return set_err(c, HCS_ERR_ARG, "invalid taxel sample lists");
+  for (int i = 0; i < S->ntaxels; ++i)
+    if (S->taxel_ptr[i + 1] < S->taxel_ptr[i]) return set_err(c, HCS_ERR_ARG, "taxel_ptr must be non-decreasing");
+  for (int k = 0; k < nnz; ++k)
+    if (S->sample_idx[k] < 0 || S->sample_idx[k] >= S->nsamples)
+      return set_err(c, HCS_ERR_ARG, "sample index out of range");
+  if (!c->last_valid)
+    return set_err(c, HCS_ERR_STATE, "hcs_curved_sensor_update needs a contact step run with keep_polygons = 1");
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  TaxelSensorBufs& B = c->ts;
+  unsigned ntri = 0;
+  if ((rc = sensor_fans(c, S->geom, ntri))) return rc;
+  if (ntri == 0) {
+    /* no contact surface touches the sensor (curved_sensor.cpp:434-438) */
+    for (int i = 0; i < S->ntaxels; ++i) values[i] = 0.0;
+    return HCS_OK;
+  }
+  const int m = S->nsamples;
+  const size_t m1 = (size_t)std::max(1, m), n1 = (size_t)S->ntaxels, z1 = (size_t)std::max(1, nnz);
+  if ((rc = ensure(c, B.P, sizeof(double) * 6 * m1)) || (rc = ensure(c, B.E, sizeof(double) * m1)) ||
+      (rc = ensure(c, B.mask, m1)) || (rc = ensure(c, B.taxel, sizeof(double) * z1)) ||
+      (rc = ensure(c, B.val, sizeof(double) * n1)) || (rc = ensure(c, B.scnt, sizeof(int) * (n1 + 1))) ||
+      (rc = ensure(c, B.soff, sizeof(int) * z1)))
+    return rc;
+  double* pts = (double*)B.P.ptr;
+  rc = hip_check(c, hipMemcpyAsync(pts, S->points, sizeof(double) * 3 * m, hipMemcpyHostToDevice, s), "ul");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(pts + 3 * m1, S->normals, sizeof(double) * 3 * m, hipMemcpyHostToDevice, s), "ul");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(B.scnt.ptr, S->taxel_ptr, sizeof(int) * (n1 + 1), hipMemcpyHostToDevice, s), "ul");
+  if (!rc && nnz > 0)
+    rc = hip_check(c, hipMemcpyAsync(B.soff.ptr, S->sample_idx, sizeof(int) * nnz, hipMemcpyHostToDevice, s), "ul");
+  if (!rc && nnz > 0)
+    rc = hip_check(c, hipMemcpyAsync(B.taxel.ptr, S->weights, sizeof(double) * nnz, hipMemcpyHostToDevice, s), "ul");
+  if (rc) return rc;
+  CsArgs a;
+  std::memcpy(a.X, S->X_W_sensor, sizeof a.X);
+  a.margin = S->include_margin;
+  if (m > 0)
+    hipLaunchKernelGGL(k_cs_rays, dim3((m + 255) / 256), dim3(256), 0, s, (const double*)B.tris.ptr, (int64_t)ntri, a,
+                       pts, pts + 3 * m1, m, (double*)B.E.ptr, (unsigned char*)B.mask.ptr);
+  hipLaunchKernelGGL(k_cs_taxels, dim3((S->ntaxels + 255) / 256), dim3(256), 0, s, (const int*)B.scnt.ptr,
+                     (const int*)B.soff.ptr, (const double*)B.taxel.ptr, S->ntaxels, (const double*)B.E.ptr,
+                     (const unsigned char*)B.mask.ptr, (double*)B.val.ptr);
+  rc = hip_check(c, hipGetLastError(), "curved sensor");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(values, B.val.ptr, sizeof(double) * n1, hipMemcpyDeviceToHost, s), "dl");
   if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
   return rc;
 }

@@ -230,6 +230,22 @@ class Engine:
         self._chk(self.lib.hcs_taxel_sensor_update(self.h, C.byref(s), A.dptr(vals), C.byref(n)))
         return vals, n.value
 
+    def curved_sensor(self, geom, X_W_sensor, points, normals, taxel_ptr, sample_idx, weights, include_margin):
+        """CurvedSensor::internal_update (S/src/curved_sensor.cpp:388-481) on
+        the polygons of the preceding contact_step(keep_polygons=True)."""
+        P, N = A.as_f64(points, (-1, 3)), A.as_f64(normals, (-1, 3))
+        ptr, idx, w = A.as_i32(taxel_ptr), A.as_i32(sample_idx), A.as_f64(weights)
+        s = A.CurvedSensor()
+        s.geom, s.ntaxels, s.nsamples = int(geom), ptr.shape[0] - 1, P.shape[0]
+        s.include_margin = float(include_margin)
+        for i, v in enumerate(A.as_f64(X_W_sensor)):
+            s.X_W_sensor[i] = v
+        s.points, s.normals = A.dptr(P), A.dptr(N)
+        s.taxel_ptr, s.sample_idx, s.weights = A.i32ptr(ptr), A.i32ptr(idx), A.dptr(w)
+        vals = np.zeros(max(0, ptr.shape[0] - 1))
+        self._chk(self.lib.hcs_curved_sensor_update(self.h, C.byref(s), A.dptr(vals)))
+        return vals
+
     # ---- differentiation
     def jacobian(self, pairs, representation=KTRIANGLE, apply_forces=True, dt=0.001):
         n = len(pairs)

@@ -1071,6 +1071,76 @@ int64_t orc_taxel_sensor(const double* tris, const int32_t* surf, int64_t ntri, 
   return m;
 }
 
+/* Moeller-Trumbore of S/src/bvh.cpp:49-74 in FP64 (|det| < 1e-10 parallel;
+ * u, v in the triangle; t > 0); returns t or -1. */
+static double mt_ray(const double* O, const double* D, const double* v0, const double* v1, const double* v2, double* uo,
+                     double* vo) {
+  double e1[3], e2[3], h[3], sv[3], qv[3];
+  sub3(v1, v0, e1);
+  sub3(v2, v0, e2);
+  cross3(D, e2, h);
+  const double a = dot3(e1, h);
+  if (std::fabs(a) < 1e-10) return -1;
+  const double f = 1.0 / a;
+  sub3(O, v0, sv);
+  const double u = f * dot3(sv, h);
+  if (u < 0.0 || u > 1.0) return -1;
+  cross3(sv, e1, qv);
+  const double v = f * dot3(D, qv);
+  if (v < 0.0 || u + v > 1.0) return -1;
+  const double t = f * dot3(e2, qv);
+  if (!(t > 0.0)) return -1;
+  *uo = u, *vo = v;
+  return t;
+}
+
+/* CurvedSensor::internal_update (S/src/curved_sensor.cpp:388-481) in FP64:
+ * the sensor's surface samples (points and normals in the sensor frame)
+ * are moved by X_W_sensor (normals rotated, not renormalised); sample j
+ * casts a ray from p_j + n_j 1e-8 along -n_j, nearest hit over the fan
+ * triangles (first in order on equal t), accepted when 0 < t < margin,
+ * raw = Evaluate(tri, (1 - u - v, u, v)); taxel i = sum over its list of
+ * weight * raw, in list order.  With no triangles every taxel is 0. */
+int orc_curved_sensor(const double* tris, int64_t ntri, const double* X, const double* pts, const double* nrms, int m,
+                      const int32_t* ptr, const int32_t* idx, const double* w, int ntaxels, double margin,
+                      double* values) {
+  if (ntri == 0) {
+    for (int i = 0; i < ntaxels; ++i) values[i] = 0.0;
+    return 0;
+  }
+  std::vector<double> raw(m, 0.0);
+  std::vector<char> hit(m, 0);
+  for (int j = 0; j < m; ++j) {
+    double pw[3], nw[3], O[3], D[3];
+    xform(X, pts + 3 * j, pw);
+    rot(X, nrms + 3 * j, nw);
+    for (int c = 0; c < 3; ++c) O[c] = pw[c] + nw[c] * 1e-8, D[c] = -nw[c];
+    double best = 1e30, bu = 0, bv = 0;
+    int64_t bt = -1;
+    for (int64_t t = 0; t < ntri; ++t) {
+      const double* T = tris + 12 * t;
+      double u, v;
+      const double tt = mt_ray(O, D, T, T + 3, T + 6, &u, &v);
+      if (tt > 0 && tt < best) best = tt, bu = u, bv = v, bt = t;
+    }
+    if (bt >= 0 && best < margin && best > 0.0) {
+      const double* T = tris + 12 * bt;
+      const double b0 = 1 - bu - bv;
+      raw[j] = (b0 * T[9] + bu * T[10]) + bv * T[11];
+      hit[j] = 1;
+    }
+  }
+  int hits = 0;
+  for (int i = 0; i < ntaxels; ++i) {
+    double pr = 0.0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k)
+      if (hit[idx[k]]) pr += w[k] * raw[idx[k]];
+    values[i] = pr;
+  }
+  for (int j = 0; j < m; ++j) hits += hit[j];
+  return hits;
+}
+
 /* The first n values of MinstdUniform (for pinning it against libstdc++). */
 void orc_minstd_uniform(int n, double* out) {
   MinstdUniform g;

@@ -114,6 +114,11 @@ int64_t orc_taxel_sensor(const double* tris, const int32_t* surf, int64_t ntri, 
                          const double* taxels, int ntaxels, double margin, double sample_resolution, int method,
                          int sample_method, int visualize, double* values);
 void orc_minstd_uniform(int n, double* out);
+/* CurvedSensor update over fan triangles; taxel lists in CSR form (ptr,
+ * idx, w); every value written.  Returns the number of rays that hit. */
+int orc_curved_sensor(const double* tris, int64_t ntri, const double* X_W_sensor, const double* pts,
+                      const double* nrms, int m, const int32_t* ptr, const int32_t* idx, const double* w,
+                      int ntaxels, double margin, double* values);
 
 /* Soft-soft contact step (tet-tet equal-pressure planes).
  *  pose = [X_WA(12), X_WB(12), V_WA(6), V_WB(6)]

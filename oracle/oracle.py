@@ -55,6 +55,8 @@ def lib():
         L.orc_taxel_sensor.argtypes = [_D, _I32, C.c_int64, _D, _D, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                        C.c_int, _D]
         L.orc_minstd_uniform.argtypes = [C.c_int, _D]
+        L.orc_curved_sensor.restype = C.c_int
+        L.orc_curved_sensor.argtypes = [_D, C.c_int64, _D, _D, _D, C.c_int, _I32, _I32, _D, C.c_int, C.c_double, _D]
         L.orc_soft_prepare.restype = C.c_void_p
         L.orc_soft_prepare.argtypes = [_D, C.c_int, _I32, C.c_int, _D]
         L.orc_soft_free.argtypes = [C.c_void_p]
@@ -278,3 +280,16 @@ def minstd_uniform(n):
     out = np.zeros(n)
     lib().orc_minstd_uniform(int(n), dp(out))
     return out
+
+
+def curved_sensor(tris, X_W_sensor, points, normals, taxel_ptr, sample_idx, weights, include_margin):
+    """CurvedSensor::internal_update (S/src/curved_sensor.cpp:388-481) over
+    fan triangles; returns (values, rays that hit)."""
+    tris, X = _d(tris).reshape(-1, 12), _d(X_W_sensor)
+    P, N = _d(points).reshape(-1, 3), _d(normals).reshape(-1, 3)
+    ptr, idx, w = _i(taxel_ptr), _i(sample_idx), _d(weights)
+    n = ptr.shape[0] - 1
+    vals = np.zeros(n)
+    hits = lib().orc_curved_sensor(dp(tris), tris.shape[0], dp(X), dp(P), dp(N), P.shape[0], ip(ptr), ip(idx), dp(w),
+                                   n, float(include_margin), dp(vals))
+    return vals, int(hits)

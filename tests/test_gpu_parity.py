@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 from mujoco_contact_surfaces_amd import scenes as S
-from meshes import pose
+from meshes import pose, rot
 
 pytestmark = pytest.mark.gpu
 
@@ -546,3 +546,55 @@ def test_taxel_sensor_matches_oracle(engine, method, sample_method, visualize):
     np.testing.assert_allclose(vals, rvals, rtol=1e-10, atol=1e-12 * np.abs(rvals).max())
     if method == "closest":
         np.testing.assert_array_equal(vals, rvals)
+
+
+# -------------------------------------------------------------- curved sensor
+
+
+def _pad_samples(sc, X_W_sensor, n=24, lift=None):
+    """Surface samples of the pad (normals +z) expressed in the sensor frame
+    X_W_sensor, and 8 x 8 taxels each averaging its 3 x 3 block of samples
+    plus a shared centre sample; the last taxel has no samples."""
+    xs, zs = sc["grid"]["half_size"][0], sc["grid"]["half_size"][2]
+    g = (np.arange(n) + 0.5) / n * 2 * xs - xs
+    P = np.array([[x, y, zs] for y in g for x in g])
+    if lift is not None:
+        P[:, 2] += lift
+    R, t = X_W_sensor[:9].reshape(3, 3), X_W_sensor[9:]
+    P_S = (P - t) @ R
+    N_S = np.tile([0.0, 0.0, 1.0], (P.shape[0], 1)) @ R
+    lists = []
+    for ty in range(8):
+        for tx in range(8):
+            l = [((3 * ty + a) * n + 3 * tx + b, 1.0 / (1 + a + b)) for a in range(3) for b in range(3)]
+            l.append(((n // 2) * n + n // 2, 0.01))
+            lists.append(l)
+    lists[-1] = []
+    ptr = np.concatenate([[0], np.cumsum([len(l) for l in lists])]).astype(np.int32)
+    idx = np.array([j for l in lists for j, _ in l], np.int32)
+    w = np.array([x for l in lists for _, x in l], float)
+    return P_S, N_S, ptr, idx, w
+
+
+@pytest.mark.parametrize("frame,lifted", [("pad", False), ("rotated", False), ("rotated", True)])
+def test_curved_sensor_matches_oracle(engine, frame, lifted):
+    say("curved sensor", frame, lifted)
+    sc = S.tactile_pad(1, L=3, K=3)
+    ha, hb = add_scene(engine, sc)
+    other = S.sphere_box(2)
+    hc, hd = add_scene(engine, other)
+    engine.contact_step([pair_of(sc, ha, hb), pair_of(other, hc, hd)], keep_polygons=True)
+    X = sc["X_WB"] if frame == "pad" else np.concatenate([rot([0.2, -0.5, 0.4], 0.6).reshape(9), [0.01, -0.02, 0.03]])
+    rng = np.random.default_rng(5)
+    lift = rng.uniform(0, 0.004, 24 * 24) if lifted else None
+    P, N, ptr, idx, w = _pad_samples(sc, X, lift=lift)
+    margin = 0.003
+    vals = engine.curved_sensor(hb, X, P, N, ptr, idx, w, margin)
+    ref = oracle_soft_rigid(sc)
+    tris, _ = O.fan_triangles(ref["polys"], np.zeros(len(ref["polys"]), int))
+    rvals, hits = O.curved_sensor(tris, X, P, N, ptr, idx, w, margin)
+    assert hits > 0 and (rvals > 0).any() and (rvals == 0).any()
+    np.testing.assert_allclose(vals, rvals, rtol=1e-10, atol=1e-12 * np.abs(rvals).max())
+    assert vals[-1] == 0.0
+    # a geometry no kept surface touches reads zero everywhere
+    np.testing.assert_array_equal(engine.curved_sensor(hc + 100, X, P, N, ptr, idx, w, margin), 0.0)

@@ -590,3 +590,105 @@ def test_area_importance_samples_about_one_per_resolution_step():
     _, m = O.taxel_sensor(tris, surf, pose(), np.zeros((1, 3)), 0.01, res, sample_method="area_importance")
     # per surface the accumulator steps through [0, total) in res * total steps
     assert 2 * 20 <= m <= 2 * 21
+
+
+# -------------------------------------------------------------- curved sensor
+
+
+def _csr(lists):
+    ptr = np.concatenate([[0], np.cumsum([len(l) for l in lists])]).astype(np.int32)
+    idx = np.array([j for l in lists for j, _ in l], np.int32)
+    w = np.array([x for l in lists for _, x in l], float)
+    return ptr, idx, w
+
+
+def test_curved_sensor_reads_a_linear_pressure_field_under_each_sample():
+    # a large triangle at z = 0 carrying f(x, y) = 100 + 2000 x - 500 y at its
+    # vertices: a ray along -z from above (x, y) reads f(x, y) exactly
+    V = np.array([[-0.1, -0.1, 0.0], [0.2, -0.1, 0.0], [-0.1, 0.2, 0.0]])
+    f = lambda x, y: 100 + 2000 * x - 500 * y
+    tri = _tri_rows(V, [f(*v[:2]) for v in V])
+    rng = np.random.default_rng(11)
+    xy = rng.uniform(-0.03, 0.03, (6, 2))
+    d = np.array([0.001, 0.0005, 0.0019, 0.003, 0.0001, 0.0025])  # margin 0.002: samples 3 and 5 miss
+    P = np.c_[xy, d]
+    N = np.tile([0.0, 0.0, 1.0], (6, 1))
+    lists = [[(0, 0.3), (1, 0.7)], [(3, 2.0)], [], [(2, 1.0), (4, -0.5), (5, 4.0), (2, 0.25)]]
+    ptr, idx, w = _csr(lists)
+    vals, hits = O.curved_sensor(tri, pose(), P, N, ptr, idx, w, 0.002)
+    assert hits == 4
+    raw = [f(*xy[j]) if d[j] < 0.002 else 0.0 for j in range(6)]
+    want = [sum(x * raw[j] for j, x in l) for l in lists]
+    np.testing.assert_allclose(vals, want, rtol=1e-12, atol=1e-12)
+    assert vals[2] == 0.0
+
+
+def test_curved_sensor_takes_the_nearest_surface_along_the_inward_normal():
+    big = np.array([[-1.0, -1.0, 0.0], [2.0, -1.0, 0.0], [-1.0, 2.0, 0.0]])
+    upper = _tri_rows(big, [5.0, 5.0, 5.0])
+    lower = _tri_rows(big - [0, 0, 0.001], [9.0, 9.0, 9.0])
+    P = np.array([[0.1, 0.1, 0.0005], [0.1, 0.1, -0.0005], [0.1, 0.1, -0.0005], [0.2, 0.1, 0.0],
+                  [0.1, 0.1, 0.0005]])
+    N = np.array([[0, 0, 1.0], [0, 0, 1.0], [0, 0, -1.0], [0, 0, 1.0], [0, 0, -1.0]])
+    ptr, idx, w = _csr([[(j, 1.0)] for j in range(5)])
+    for tris in (np.concatenate([upper, lower]), np.concatenate([lower, upper])):
+        vals, hits = O.curved_sensor(tris, pose(), P, N, ptr, idx, w, 0.01)
+        # above both: the upper one; between, facing up: the lower one;
+        # between, facing down (ray along +z): the upper one; on the upper
+        # surface: the ray starts 1e-8 above it and hits it; above both,
+        # facing down: the ray leaves both behind
+        np.testing.assert_array_equal(vals, [5.0, 9.0, 5.0, 5.0, 0.0])
+        assert hits == 4
+    vals, _ = O.curved_sensor(lower, pose(), P, N, ptr, idx, w, 0.01)
+    np.testing.assert_array_equal(vals, [9.0, 9.0, 0.0, 9.0, 0.0])
+
+
+def test_curved_sensor_without_surfaces_reads_zero():
+    ptr, idx, w = _csr([[(0, 1.0)], []])
+    vals, hits = O.curved_sensor(np.zeros((0, 12)), pose(), np.zeros((1, 3)), np.array([[0, 0, 1.0]]), ptr, idx, w, 1.0)
+    np.testing.assert_array_equal(vals, [0.0, 0.0])
+    assert hits == 0
+
+
+def test_curved_sensor_hits_agree_with_a_linear_solve():
+    """Moeller-Trumbore against O + t D = v0 + u e1 + v e2 solved by LAPACK:
+    e = (0, 1, 0) makes the reading u, e = (0, 0, 1) makes it v."""
+    rng = np.random.default_rng(13)
+    checked = 0
+    for _ in range(600):
+        V = rng.uniform(-1, 1, (3, 3))
+        n = rng.normal(size=3)
+        a, b = rng.uniform(-0.3, 1.0, 2)
+        p = V[0] + a * (V[1] - V[0]) + b * (V[2] - V[0]) + rng.uniform(-0.5, 1.5) * n
+        O_, D = p + n * 1e-8, -n
+        t, u, v = np.linalg.solve(np.c_[-D, V[1] - V[0], V[2] - V[0]], O_ - V[0])
+        if min(abs(u), abs(v), abs(1 - u - v)) < 1e-6 or abs(t) < 1e-6:
+            continue  # too close to an edge or the origin to call
+        inside = u > 0 and v > 0 and u + v < 1 and t > 0
+        ptr, idx, w = _csr([[(0, 1.0)]])
+        ru, h1 = O.curved_sensor(_tri_rows(V, [0, 1, 0]), pose(), p[None], n[None], ptr, idx, w, 10.0)
+        rv, _ = O.curved_sensor(_tri_rows(V, [0, 0, 1]), pose(), p[None], n[None], ptr, idx, w, 10.0)
+        assert h1 == int(inside and t < 10.0)
+        if h1:
+            np.testing.assert_allclose([ru[0], rv[0]], [u, v], rtol=1e-9, atol=1e-12)
+            # the margin test is on t: just above t accepts, just below rejects
+            assert O.curved_sensor(_tri_rows(V, [0, 1, 0]), pose(), p[None], n[None], ptr, idx, w, t * (1 + 1e-9))[1]
+            assert not O.curved_sensor(_tri_rows(V, [0, 1, 0]), pose(), p[None], n[None], ptr, idx, w,
+                                       t * (1 - 1e-9))[1]
+            checked += 1
+    assert checked > 100
+
+
+def test_curved_sensor_is_rigidly_invariant():
+    rng = np.random.default_rng(17)
+    V = rng.uniform(-0.02, 0.02, (3, 3))
+    V[:, 2] = 0.0
+    e = np.array([30.0, 60.0, 90.0])
+    P = np.c_[rng.uniform(-0.005, 0.005, (12, 2)), rng.uniform(0, 0.002, 12)]
+    N = np.tile([0.0, 0.0, 1.0], (12, 1))
+    ptr, idx, w = _csr([[(j, 1.0 + j)] for j in range(12)])
+    a, ha = O.curved_sensor(_tri_rows(V, e), pose(), P, N, ptr, idx, w, 0.003)
+    R, t = rot([0.3, -0.4, 0.8], 1.1), np.array([0.5, -0.2, 0.1])
+    b, hb = O.curved_sensor(_tri_rows(V @ R.T + t, e), pose(R, t), P, N, ptr, idx, w, 0.003)
+    assert ha == hb > 0
+    np.testing.assert_allclose(b, a, rtol=1e-9, atol=1e-12 * np.abs(a).max())
