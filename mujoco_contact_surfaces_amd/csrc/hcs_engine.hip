@@ -461,7 +461,7 @@ __global__ void k_split_roots(const EnvDesc* __restrict__ envs, const Item* __re
 /* Candidate generation: one wave per item; persistent when a work-queue
  * counter is given (the counter is reset by a fill kernel in the same
  * graph). */
-template <int MODE>
+template <int MODE, int KS>
 __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
   unsigned long long s_next = 0, s_end = 0; /* this wave's survivor slots */
@@ -475,9 +475,9 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
       if (v < n) {
         if (A.live) {
           const int4 r = A.live_item[A.qseg[x] + v];
-          run_cand<MODE>(L, A, A.live[A.qseg[x] + v], Item{r.x, r.y, r.z, r.w}, s_next, s_end);
+          run_cand<MODE, KS>(L, A, A.live[A.qseg[x] + v], Item{r.x, r.y, r.z, r.w}, s_next, s_end);
         } else {
-          run_cand<MODE>(L, A, (int)v, A.items[v], s_next, s_end);
+          run_cand<MODE, KS>(L, A, (int)v, A.items[v], s_next, s_end);
         }
         break;
       }
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
           nrec = A.live_item[base + vn];
           if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
         }
-        run_cand<MODE>(L, A, idx, Item{rec.x, rec.y, rec.z, rec.w}, s_next, s_end);
+        run_cand<MODE, KS>(L, A, idx, Item{rec.x, rec.y, rec.z, rec.w}, s_next, s_end);
         if (vn >= n) break;
         idx = nidx;
         rec = nrec;
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
       if (lane == 0) v = atomicAdd(A.counter, 1u);
       v = __shfl(v, 0, 64);
       if (v >= (unsigned)A.nitems) break;
-      run_cand<MODE>(L, A, (int)v, A.items[v], s_next, s_end);
+      run_cand<MODE, KS>(L, A, (int)v, A.items[v], s_next, s_end);
     }
   }
   fill_sentinels(A, s_next, s_end);
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
 
 /* Clip, faces and force of every survivor (grid-stride); grid.y selects the
  * tangent direction in the Dual instantiation. */
-template <class T>
+template <class T, int KS>
 __global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int dir_base, Contrib* out,
                                               int64_t out_stride) {
   const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int di
   const unsigned long long cnt = *A.surv_cnt;
   const int64_t n = (int64_t)cnt < A.surv_cap ? (int64_t)cnt : A.surv_cap;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    clip_survivor<T>(A, C, i, dir, dst, op);
+    clip_survivor<T, KS>(A, C, i, dir, dst, op);
 }
 
 /* Per pair (one wave, grid.y = direction): its survivors' contributions
@@ -632,6 +632,7 @@ QueryArgs pipe_args(const Pipe& p, const EnvDesc* envs, const double* state, con
   a.nqueues = p.nqueues;
   a.live_item = p.live.ptr ? (const int4*)p.live_item.ptr : nullptr;
   a.xtri = (const double*)p.xtri.ptr;
+  a.kinds = p.kinds;
   return a;
 }
 
@@ -653,14 +654,28 @@ void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs) {
                        a.item_ncand, a.item_nsurv);
 }
 
+/* The kind a launch is specialised for: the one kind all its pairs share,
+ * triangle or soft queries (the common single-kind batches), else -1. */
+static int launch_kind(const QueryArgs& a) {
+  if (a.kinds == (1 << KIND_TRI)) return KIND_TRI;
+  if (a.kinds == (1 << KIND_SOFT)) return KIND_SOFT;
+  return -1;
+}
+
+template <int MODE>
+static void launch_cand_mode(hipStream_t s, const QueryArgs& a, int grid) {
+  switch (launch_kind(a)) {
+    case KIND_TRI: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<MODE, KIND_TRI>), dim3(grid), dim3(64), 0, s, a); break;
+    case KIND_SOFT: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<MODE, KIND_SOFT>), dim3(grid), dim3(64), 0, s, a); break;
+    default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<MODE, -1>), dim3(grid), dim3(64), 0, s, a); break;
+  }
+}
+
 void launch_cand(hipStream_t s, const QueryArgs& a, int grid, int mode) {
   if (a.nitems <= 0 || grid <= 0) return;
-  if (mode == CAND_KEYS)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<CAND_KEYS>), dim3(grid), dim3(64), 0, s, a);
-  else if (mode == CAND_COUNT)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<CAND_COUNT>), dim3(grid), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cand<CAND_STEP>), dim3(grid), dim3(64), 0, s, a);
+  if (mode == CAND_KEYS) launch_cand_mode<CAND_KEYS>(s, a, grid);
+  else if (mode == CAND_COUNT) launch_cand_mode<CAND_COUNT>(s, a, grid);
+  else launch_cand_mode<CAND_STEP>(s, a, grid);
 }
 
 /* Scan of the per-item survivor counts, [event e_clip], clip (double, or
@@ -674,12 +689,19 @@ static void pipe_back(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_
   if (e_clip) (void)hipEventRecord(e_clip, s);
   ClipArgs C{(const unsigned*)p.off.ptr, dstar_dE, invE, p.cap};
   const int g = std::max(1, clip_grid);
-  if (ndir <= 0)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double>), dim3(g), dim3(256), 0, s, a, C, -1, (Contrib*)p.contrib.ptr,
-                       (int64_t)0);
-  else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<Dual>), dim3(g, ndir), dim3(256), 0, s, a, C, dir0,
+  const int kind = launch_kind(a);
+  if (ndir > 0)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<Dual, -1>), dim3(g, ndir), dim3(256), 0, s, a, C, dir0,
                        (Contrib*)p.contrib.ptr, (int64_t)p.cap);
+  else if (kind == KIND_TRI)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double, KIND_TRI>), dim3(g), dim3(256), 0, s, a, C, -1,
+                       (Contrib*)p.contrib.ptr, (int64_t)0);
+  else if (kind == KIND_SOFT)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double, KIND_SOFT>), dim3(g), dim3(256), 0, s, a, C, -1,
+                       (Contrib*)p.contrib.ptr, (int64_t)0);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double, -1>), dim3(g), dim3(256), 0, s, a, C, -1,
+                       (Contrib*)p.contrib.ptr, (int64_t)0);
   if (e_reduce) (void)hipEventRecord(e_reduce, s);
   if (npairs > 0)
     hipLaunchKernelGGL(k_env_reduce, dim3(npairs, std::max(1, ndir)), dim3(64), 0, s, a.envs,
@@ -719,6 +741,8 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
   for (int x = 0; x < HCS_QUEUES; ++x) seg[x + 1] += seg[x];
   p.qseg_host = seg;
   p.nxenv = (int)plan.xenv.size();
+  p.kinds = 0;
+  for (const EnvDesc& E : plan.envs) p.kinds |= 1 << E.kind;
   p.ordered = (int)plan.order.size() == nitems && nitems > 0;
   if (p.ordered) {
     if ((rc = ensure(c, p.order, sizeof(int) * ni))) return rc;

@@ -47,6 +47,7 @@ struct Pipe {
   int nqueues = 1;            /* candidate-kernel work queues (1..HCS_QUEUES) */
   int nxenv = 0;              /* pairs in tet-query mode (their triangles are moved per step) */
   bool ordered = false;       /* order holds a work order of the items */
+  int kinds = 0;              /* bit k set: some pair of the plan is of EnvKind k (0: not known) */
 };
 
 struct Ctx;

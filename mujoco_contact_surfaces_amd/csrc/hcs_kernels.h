@@ -131,6 +131,9 @@ struct QueryArgs {
   int32_t nqueues;
   const int4* live_item; /* the live items' Item records, same order as live */
   const double* xtri;    /* per-step moved triangles of tet-query pairs */
+  int32_t kinds;         /* bit k set: some pair is of EnvKind k; a launch
+                            whose pairs share one kind runs a kernel built
+                            for it (0: not known, the general kernel) */
 };
 
 /* Inputs of the clip kernel beyond QueryArgs. */

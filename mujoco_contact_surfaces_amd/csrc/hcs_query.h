@@ -321,13 +321,16 @@ __device__ inline bool slice_tet(const EnvDesc& E, int a, const T* nh, T off, Po
   return true;
 }
 
-template <class T>
+/* KS: the pair kind when every pair of the launch has it (the kind branches
+ * fold away), or -1 for a launch that mixes kinds. */
+template <class T, int KS = -1>
 __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic, int env, int a, int b, Acc<T>& acc,
                                          int64_t& np, int64_t& nf, const OutPtrs* o) {
+  const int kind = KS >= 0 ? KS : E.kind;
   const double* __restrict__ rr = E.recA + (size_t)HCS_TREC * a;
   T gA[3] = {ic.escale * rr[16], ic.escale * rr[17], ic.escale * rr[18]};
   T cA = ic.escale * rr[19];
-  if (E.kind == KIND_PLANE) {
+  if (kind == KIND_PLANE) {
     /* soft-half-space: B's plane z = 0 (normal +z, out of the rigid side)
      * in A's frame; the gradient filter of rigid faces, then the slice */
     const T ez[3] = {T(0.0), T(0.0), T(1.0)};
@@ -340,7 +343,7 @@ __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic,
     ++np;
     T zero3[3] = {T(0.0), T(0.0), T(0.0)};
     emit_polygon<T, 4>(ic, p, nS, gA, cA, zero3, acc, nf, o, env, a, b);
-  } else if (E.kind != KIND_SOFT) {
+  } else if (kind != KIND_SOFT) {
     /* soft-rigid: triangle b of the rigid mesh moved into A's frame */
     const int32_t* f = E.FB + 3 * (size_t)b;
     Poly<T> p;
@@ -454,10 +457,11 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
  *    the tet.
  * soft-soft: the equal-pressure plane, both gradient filters and the
  * marching-tets sign test of process_candidate (same arithmetic). */
-template <class T>
+template <int KS = -1>
 __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b, const double* tv, const double* tn) {
+  const int kind = KS >= 0 ? KS : E.kind;
   const double* r = E.recA + (size_t)HCS_TREC * a;
-  if (E.kind == KIND_PLANE) {
+  if (kind == KIND_PLANE) {
     /* the gradient filter and the slice's sign test of process_candidate */
     const double ez[3] = {0.0, 0.0, 1.0};
     double nS[3];
@@ -470,7 +474,7 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
     for (int i = 0; i < 4; ++i) above += (dot3(nS, E.VA + 3 * (size_t)ta[i]) + off) > 0;
     return above != 0 && above != 4;
   }
-  if (E.kind != KIND_SOFT) {
+  if (kind != KIND_SOFT) {
     /* tv, tn: triangle b's vertices and unit normal already moved into A's
      * frame (xform / rot with X, the same arithmetic as process_candidate) */
     const double nS[3] = {tn[0], tn[1], tn[2]};
@@ -588,17 +592,34 @@ __device__ inline bool box_may_meet_plane(const double* X, const double* Q) {
 #define HCS_LSTACK 16
 #endif
 
+/* 13.1 KB, so 12 waves fit in a CU's 160 KB of LDS.  Query boxes are not
+ * stored: a lane keeps its own in registers, and a box is rebuilt from the
+ * vertices in tv / tn (min / max, exact) when a lane steals a subtree or a
+ * tile triangle is tested. */
 struct CandLds {
   int stk[HCS_LSTACK * 64];
   int2 q1[192];
-  int2 q2[128];
-  double box[64 * 6];
-  double tv[64 * 9]; /* query / tile triangles in A's frame */
-  double tn[64 * 3]; /* their unit normals in A's frame */
+  int2 q2[128]; /* while producing, q2 holds fewer than 64 entries and
+                   q2[64..127] carries the work-stealing hand-over */
+  double tv[64 * 9]; /* query / tile triangles (first three vertices of a
+                        soft query tet) in A's frame */
+  double tn[64 * 3]; /* triangles: unit normals in A's frame; soft queries:
+                        the tet's fourth vertex */
   int qb[64];        /* triangle / soft queries: the element of each slot */
-  int2 xfer[64];
   double X[12];
 };
+
+/* The axis-aligned box of LDS slot k: its triangle, or with the fourth
+ * vertex its soft query tet. */
+__device__ inline void slot_box(const CandLds& L, int k, bool soft, double* lo, double* hi) {
+  const double* v = &L.tv[9 * k];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    lo[c] = fmin(fmin(v[c], v[3 + c]), v[6 + c]);
+    hi[c] = fmax(fmax(v[c], v[3 + c]), v[6 + c]);
+    if (soft) lo[c] = fmin(lo[c], L.tn[3 * k + c]), hi[c] = fmax(hi[c], L.tn[3 * k + c]);
+  }
+}
 
 /* Survivor slots are reserved per wave in chunks of this many (one atomic
  * per chunk); slots a wave leaves unused are filled with sentinels. */
@@ -617,7 +638,7 @@ __device__ inline void fill_sentinels(const QueryArgs& A, unsigned long long fro
  * place each contribution at item_off[item] + seq whatever order the groups
  * land in.  CAND_KEYS writes the canonical candidate keys instead and
  * CAND_COUNT only counts candidates per item (no tests, no survivors). */
-template <int MODE>
+template <int MODE, int KS = -1>
 __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const Item item,
                                 unsigned long long& s_next, unsigned long long& s_end) {
   const int lane = threadIdx.x & 63;
@@ -633,8 +654,9 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
   int nc = 0;
   unsigned nsurv = 0;
   int q1n = 0, q2n = 0;
-  const bool tet_mode = E.kind == KIND_TET;
-  const bool plane_mode = E.kind == KIND_PLANE;
+  const int kind = KS >= 0 ? KS : E.kind;
+  const bool tet_mode = kind == KIND_TET;
+  const bool plane_mode = kind == KIND_PLANE;
   bool producing = true;
   /* traversal state (triangle / soft queries) */
   int ovf[HCS_STACK - HCS_LSTACK];
@@ -693,7 +715,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
     const int e = E.qorder ? E.qorder[item.first + lane < E.nB ? item.first + lane : 0] : item.first + lane;
     L.qb[lane] = e;
     if (valid) {
-      const int nvq = E.kind == KIND_SOFT ? 4 : 3;
+      const int nvq = kind == KIND_SOFT ? 4 : 3;
       const int32_t* f = E.FB + (size_t)nvq * e;
       for (int i = 0; i < nvq; ++i) {
         double y[3];
@@ -701,6 +723,9 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
         if (i < 3) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) L.tv[9 * lane + 3 * i + c] = y[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = y[c];
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -708,14 +733,12 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
           hi[c] = i == 0 ? y[c] : fmax(hi[c], y[c]);
         }
       }
-      if (E.kind != KIND_SOFT) {
+      if (kind != KIND_SOFT) {
         double nS[3];
         rot(X, E.nrmB + 3 * (size_t)e, nS);
 #pragma unroll
         for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
       }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = lo[c], L.box[6 * lane + 3 + c] = hi[c];
     }
     if (E.ntA == 1 || item.root < 0) {
       /* a one-tet mesh, or a sub-item whose subtree is a single leaf */
@@ -774,8 +797,6 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
 #pragma unroll
                 for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
               }
-#pragma unroll
-              for (int c = 0; c < 3; ++c) L.box[6 * lane + c] = l3[c], L.box[6 * lane + 3 + c] = h3[c];
               const double ub[6] = {ulo[0], ulo[1], ulo[2], uhi[0], uhi[1], uhi[2]};
               touch = box_overlap(ub, l3, h3);
             }
@@ -785,8 +806,9 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
         } else {
           const int jj = __builtin_ctzll(tmask);
           tmask &= tmask - 1;
-          const double* tb = &L.box[6 * jj];
-          const bool hit = valid && box_overlap(bx, tb, tb + 3);
+          double tl[3], th[3];
+          slot_box(L, jj, false, tl, th);
+          const bool hit = valid && box_overlap(bx, tl, th);
           const unsigned long long m = __ballot(hit);
           if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, t0 + jj);
           q1n += __popcll(m);
@@ -803,10 +825,11 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
           const unsigned long long idle = __ballot(cnt == 0), donors = __ballot(donor);
           const int npair = min(__popcll(idle), __popcll(donors));
           if (npair > 0) {
+            int2* xfer = L.q2 + 64;
             if (donor) {
               const int r = (int)lane_prefix(donors);
               if (r < npair) {
-                L.xfer[r] = make_int2(b, L.stk[(bp % HCS_LSTACK) * 64 + lane]);
+                xfer[r] = make_int2(b, L.stk[(bp % HCS_LSTACK) * 64 + lane]);
                 ++bp;
                 --cnt;
               }
@@ -815,10 +838,9 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
             if (cnt == 0) {
               const int r = (int)lane_prefix(idle);
               if (r < npair) {
-                const int2 w = L.xfer[r];
+                const int2 w = xfer[r];
                 b = w.x;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) lo[c] = L.box[6 * w.x + c], hi[c] = L.box[6 * w.x + 3 + c];
+                slot_box(L, w.x, kind == KIND_SOFT, lo, hi);
                 L.stk[lane] = w.y;
                 bp = 0, sp = 1, on = 0;
                 cnt = 1;
@@ -914,7 +936,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
             unsigned long long k = atomicAdd(A.cand_cnt, 1ull);
             if ((int64_t)k < A.cand_cap) A.cand_key[k] = (int64_t)c.x * E.nB + c.y;
           } else if (MODE == CAND_STEP) {
-            keep = survives<double>(E, X, c.x, c.y, &L.tv[9 * slot], &L.tn[3 * slot]);
+            keep = survives<KS>(E, X, c.x, c.y, &L.tv[9 * slot], &L.tn[3 * slot]);
           }
         }
         q1n = base;
@@ -948,7 +970,7 @@ struct Contrib {
   int np, nf;
 };
 
-template <class T>
+template <class T, int KS = -1>
 __device__ inline void clip_survivor(const QueryArgs& A, const ClipArgs& C, int64_t i, int dir, Contrib* out,
                                      const OutPtrs* o) {
   const int4 s = A.surv[i];
@@ -963,7 +985,7 @@ __device__ inline void clip_survivor(const QueryArgs& A, const ClipArgs& C, int6
   Acc<T> acc;
   acc.zero();
   int64_t np = 0, nf = 0;
-  process_candidate<T>(E, ic, env, s.x, s.y, acc, np, nf, o);
+  process_candidate<T, KS>(E, ic, env, s.x, s.y, acc, np, nf, o);
   Contrib r;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {

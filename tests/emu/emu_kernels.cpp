@@ -247,7 +247,11 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
         g_lane = lane;
         threadIdx.x = lane;
         unsigned long long sn = 0, se = 0;
-        run_cand<CAND_STEP>(*L, Q, it, items[it], sn, se);
+        /* the kernels the engine launches: built for the pair's kind for
+         * triangle and soft queries, the general one otherwise */
+        if (E.kind == KIND_TRI) run_cand<CAND_STEP, KIND_TRI>(*L, Q, it, items[it], sn, se);
+        else if (E.kind == KIND_SOFT) run_cand<CAND_STEP, KIND_SOFT>(*L, Q, it, items[it], sn, se);
+        else run_cand<CAND_STEP>(*L, Q, it, items[it], sn, se);
         fill_sentinels(Q, sn, se);
       });
     for (auto& t : th) t.join();
@@ -262,7 +266,10 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
   for (int dir = -1; dir < ndir; ++dir) {
     for (auto& q : con) q = Contrib{};
     for (unsigned long long i = 0; i < scnt; ++i) {
-      if (dir < 0) clip_survivor<double>(Q, C, (int64_t)i, dir, con.data(), nullptr);
+      if (dir < 0 && E.kind == KIND_TRI) clip_survivor<double, KIND_TRI>(Q, C, (int64_t)i, dir, con.data(), nullptr);
+      else if (dir < 0 && E.kind == KIND_SOFT)
+        clip_survivor<double, KIND_SOFT>(Q, C, (int64_t)i, dir, con.data(), nullptr);
+      else if (dir < 0) clip_survivor<double>(Q, C, (int64_t)i, dir, con.data(), nullptr);
       else clip_survivor<Dual>(Q, C, (int64_t)i, dir, con.data(), nullptr);
     }
     double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
