@@ -407,6 +407,44 @@ __device__ inline bool box_overlap(const double* a /*lo,hi*/, const double* lo, 
   return a[0] <= hi[0] && lo[0] <= a[3] && a[1] <= hi[1] && lo[1] <= a[4] && a[2] <= hi[2] && lo[2] <= a[5];
 }
 
+/* Whether a tet inside box bx (lo, hi) can meet triangle v (three vertices)
+ * with unit normal n.  Separating axes: the triangle's normal (the same
+ * test as the last of survives()) and, per edge, the in-plane direction
+ * m = edge x n turned away from the opposite vertex, against the
+ * triangle's largest projection on m.  A box that one axis separates from
+ * the triangle by more than 1e-12 of the axis length holds no point within
+ * rounding of the triangle, so clipping the triangle by the tet leaves
+ * nothing; the axis itself may be rounded, since any axis separates. */
+__device__ inline bool box_may_meet_triangle(const double* bx, const double* v, const double* n) {
+  double c[3], h[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = 0.5 * (bx[k] + bx[3 + k]), h[k] = 0.5 * (bx[3 + k] - bx[k]);
+  {
+    double dist = -((n[0] * v[0] + n[1] * v[1]) + n[2] * v[2]);
+    double rad = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dist += n[k] * c[k], rad += fabs(n[k]) * h[k];
+    if (fabs(dist) > rad + 1e-12) return false;
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const double* a = v + 3 * e;
+    const double* b = v + 3 * ((e + 1) % 3);
+    const double* o = v + 3 * ((e + 2) % 3);
+    const double ed[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double m[3];
+    cross3(ed, n, m);
+    const double pa = dot3(m, a), pb = dot3(m, b), po = dot3(m, o);
+    const double sg = po > pa ? -1.0 : 1.0;
+    const double tmax = sg > 0.0 ? fmax(fmax(pa, pb), po) : fmax(fmax(-pa, -pb), -po);
+    double bmin = sg * dot3(m, c);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bmin -= fabs(m[k]) * h[k];
+    if (bmin - tmax > 1e-12 * sqrt(dot3(m, m))) return false;
+  }
+  return true;
+}
+
 __device__ inline unsigned lane_prefix(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
@@ -884,6 +922,13 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
               }
               L.stk[(sp % HCS_LSTACK) * 64 + lane] = ch;
               ++sp;
+            }
+            if (MODE == CAND_STEP && kind == KIND_TRI) {
+              /* leaf hits whose tet box cannot meet the triangle are
+               * candidates that the clip would leave empty: counted, not
+               * queued */
+              if (c0 >= 0 && !box_may_meet_triangle(bb, &L.tv[9 * b], &L.tn[3 * b])) c0 = -1, ++nc;
+              if (c1 >= 0 && !box_may_meet_triangle(bb + 6, &L.tv[9 * b], &L.tn[3 * b])) c1 = -1, ++nc;
             }
           }
           const unsigned long long m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
