@@ -76,7 +76,7 @@ __constant__ int kFace[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
  * arithmetic as the oracle's tet record. */
 __global__ void k_tet_precompute(const double* __restrict__ V, const int32_t* __restrict__ T,
                                  const double* __restrict__ p, int nt, double* __restrict__ rec,
-                                 double* __restrict__ box) {
+                                 double* __restrict__ box, float4* __restrict__ fpl) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nt) return;
   const int32_t* tet = T + 4 * (size_t)t;
@@ -99,6 +99,8 @@ __global__ void k_tet_precompute(const double* __restrict__ V, const int32_t* __
     double nh[3] = {n[0] / len, n[1] / len, n[2] / len};
     r[3 * k] = nh[0], r[3 * k + 1] = nh[1], r[3 * k + 2] = nh[2];
     r[12 + k] = -dot3(nh, A);
+    /* the same plane rounded to float (nearest) */
+    fpl[4 * (size_t)t + k] = make_float4((float)nh[0], (float)nh[1], (float)nh[2], (float)r[12 + k]);
   }
   double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
   double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
@@ -131,6 +133,12 @@ __global__ void k_tet_precompute(const double* __restrict__ V, const int32_t* __
     for (int j = 1; j < 4; ++j) lo = fmin(lo, v[j][c]), hi = fmax(hi, v[j][c]);
     b[c] = lo, b[3 + c] = hi;
   }
+}
+
+/* 4-wide nodes of the LBVH's n_int internal nodes. */
+__global__ void k_wide_nodes(const Node* __restrict__ nodes, int n_int, WNode* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_int) make_wide_node(nodes, i, w[i]);
 }
 
 /* Unit triangle normals (Drake's TriangleSurfaceMesh face normals). */
@@ -462,7 +470,7 @@ __global__ void k_split_roots(const EnvDesc* __restrict__ envs, const Item* __re
  * counter is given (the counter is reset by a fill kernel in the same
  * graph). */
 template <int MODE, int KS>
-__global__ __launch_bounds__(64) void k_cand(QueryArgs A) {
+__global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 1) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
   unsigned long long s_next = 0, s_end = 0; /* this wave's survivor slots */
   const int nq = A.live ? A.nqueues : 1;
@@ -874,6 +882,13 @@ static int build_bvh(Ctx* c, Mesh& m) {
     m.height = h;
   }
   if (rc == HCS_OK) {
+    fail(hipMalloc((void**)&m.wnodes, (size_t)(n - 1) * sizeof(WNode)), "hipMalloc");
+    if (rc == HCS_OK) {
+      hipLaunchKernelGGL(k_wide_nodes, dim3((n - 1 + 255) / 256), dim3(256), 0, c->stream, m.nodes, n - 1, m.wnodes);
+      fail(hipGetLastError(), "k_wide_nodes");
+    }
+  }
+  if (rc == HCS_OK) {
     fail(hipMalloc((void**)&m.boxL, 6 * sizeof(double) * (size_t)n), "hipMalloc");
     if (rc == HCS_OK) {
       hipLaunchKernelGGL(k_gather_boxes, dim3((n + 255) / 256), dim3(256), 0, c->stream, m.box, idx2, n, m.boxL);
@@ -888,14 +903,14 @@ static int build_bvh(Ctx* c, Mesh& m) {
   for (void* ptr : {(void*)codes, (void*)codes2, (void*)idx, (void*)idx2, (void*)np, (void*)lp, (void*)flags,
                     (void*)rh, tmp})
     if (ptr) (void)hipFree(ptr);
-  if (rc == HCS_OK && m.height > HCS_STACK - 2)
+  if (rc == HCS_OK && 3 * ((m.height + 1) / 2) + 1 > HCS_STACK)
     rc = set_err(c, HCS_ERR_LIMIT, "LBVH height " + std::to_string(m.height) + " exceeds the traversal stack");
   return rc;
 }
 
 static void free_mesh(Mesh& m) {
   for (void* p : {(void*)m.V, (void*)m.E, (void*)m.p, (void*)m.rec, (void*)m.box, (void*)m.nodes, (void*)m.nrm,
-                  (void*)m.leaf, (void*)m.boxL, (void*)m.qorder})
+                  (void*)m.leaf, (void*)m.boxL, (void*)m.qorder, (void*)m.fpl, (void*)m.wnodes})
     if (p) (void)hipFree(p);
   m = Mesh();
 }
@@ -918,7 +933,7 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
       return set_err(c, HCS_ERR_ARG,
                      "pair " + std::to_string(e) + ": geom_a must be soft (pass soft-rigid pairs soft first)");
     EnvDesc& E = plan.envs[e];
-    E.recA = A.rec, E.boxA = A.box, E.nodesA = A.nodes, E.leafA = A.leaf, E.boxLA = A.boxL, E.TA = A.E, E.VA = A.V,
+    E.recA = A.rec, E.fplA = (const float4*)A.fpl, E.boxA = A.box, E.nodesA = A.nodes, E.wnodesA = A.wnodes, E.leafA = A.leaf, E.boxLA = A.boxL, E.TA = A.E, E.VA = A.V,
     E.ntA = A.nelem;
     E.xtri_off = -1;
     E.VB = B.V, E.FB = B.E, E.nrmB = B.nrm, E.recB = B.rec, E.nB = B.nelem, E.nvB = B.nv;
@@ -1071,9 +1086,10 @@ int hcs_add_soft_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* T,
   if (rc == HCS_OK) rc = upload(c, (void**)&m.p, pr, sizeof(double) * nv);
   if (rc == HCS_OK) rc = hip_check(c, hipMalloc((void**)&m.rec, (size_t)HCS_TREC * sizeof(double) * nt), "hipMalloc");
   if (rc == HCS_OK) rc = hip_check(c, hipMalloc((void**)&m.box, 6 * sizeof(double) * (size_t)nt), "hipMalloc");
+  if (rc == HCS_OK) rc = hip_check(c, hipMalloc((void**)&m.fpl, 16 * sizeof(float) * (size_t)nt), "hipMalloc");
   if (rc == HCS_OK) {
     hipLaunchKernelGGL(k_tet_precompute, dim3((nt + 127) / 128), dim3(128), 0, c->stream, m.V, m.E, m.p, nt, m.rec,
-                       m.box);
+                       m.box, (float4*)m.fpl);
     rc = hip_check(c, hipGetLastError(), "k_tet_precompute");
   }
   if (rc == HCS_OK) rc = build_bvh(c, m);

@@ -22,8 +22,10 @@ struct Mesh {
   int32_t* E = nullptr; /* T (soft) or F (rigid) */
   double* p = nullptr;
   double* rec = nullptr;
+  float* fpl = nullptr; /* soft: the record's four face planes in float, 64 B per tet */
   double* box = nullptr;
   Node* nodes = nullptr;
+  WNode* wnodes = nullptr; /* soft: 4-wide nodes, one per internal LBVH node */
   int32_t* leaf = nullptr; /* tets in Morton (BVH leaf) order */
   int32_t* qorder = nullptr; /* rigid: triangles in Morton order of their centroids */
   double* boxL = nullptr;   /* their AABBs in that order */

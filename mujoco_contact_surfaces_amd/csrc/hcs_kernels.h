@@ -27,7 +27,9 @@
 namespace hcs {
 
 #define HCS_TREC 24
-#define HCS_STACK 64
+/* traversal stack entries per lane: a 4-wide walk of a binary LBVH of
+ * height h keeps at most 3 ceil(h / 2) + 1 */
+#define HCS_STACK 96
 #define HCS_FREC 12
 #define HCS_PREC 40
 #define HCS_STATE 36
@@ -47,6 +49,44 @@ struct Node {
 };
 static_assert(sizeof(Node) == 128, "one LBVH node per cache line");
 
+/* The 4-wide node of binary internal node i: its grandchildren (a leaf
+ * child stands for itself), so a walk over wide nodes visits every other
+ * level of the binary tree.  b[6k..6k+5]: child k's box (lo, hi, the
+ * binary node's double box); ch[k]: >= 0 the wide node of binary node
+ * ch[k], < 0 the leaf ~ch[k], HCS_WNONE an empty slot (whose box is
+ * empty). */
+struct WNode {
+  double b[24];
+  int32_t ch[4];
+  int32_t pad[12];
+};
+static_assert(sizeof(WNode) == 256, "one wide node per two cache lines");
+#define HCS_WNONE INT32_MIN
+
+/* Builds the wide node of binary internal node i. */
+__host__ __device__ inline void make_wide_node(const Node* nodes, int i, WNode& w) {
+  const Node& n = nodes[i];
+  int m = 0;
+  auto put = [&](const double* b, int ch) {
+    for (int c = 0; c < 6; ++c) w.b[6 * m + c] = b[c];
+    w.ch[m++] = ch;
+  };
+  for (int s = 0; s < 2; ++s) {
+    const int c = n.c[s];
+    if (c < 0) {
+      put(n.b + 6 * s, c);
+    } else {
+      const Node& g = nodes[c];
+      for (int t = 0; t < 2; ++t) put(g.b + 6 * t, g.c[t]);
+    }
+  }
+  for (; m < 4; ++m) {
+    for (int c = 0; c < 3; ++c) w.b[6 * m + c] = INFINITY, w.b[6 * m + 3 + c] = -INFINITY;
+    w.ch[m] = HCS_WNONE;
+  }
+  for (int k = 0; k < 12; ++k) w.pad[k] = 0;
+}
+
 /* query kinds: rigid triangles against A's LBVH, A's tets against a small
  * rigid mesh, tets of a soft B against A's LBVH, A's tets against the rigid
  * half-space z <= 0 of B's frame (a plane geom) */
@@ -58,8 +98,10 @@ enum CandMode { CAND_STEP = 0, CAND_KEYS = 1, CAND_COUNT = 2 };
 
 struct EnvDesc {
   const double* recA;
+  const float4* fplA; /* A's face planes rounded to float (n, d), 4 per tet */
   const double* boxA;
   const Node* nodesA;
+  const WNode* wnodesA; /* the 4-wide nodes of A's LBVH, one per internal node */
   const int32_t* leafA; /* A's tets in Morton order (tet-query items) */
   const double* boxLA;  /* their AABBs in that order */
   const int32_t* TA;

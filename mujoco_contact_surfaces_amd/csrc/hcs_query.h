@@ -483,16 +483,18 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
   return v;
 }
 
-/* Exact early rejection, evaluated on primal values before the full clip.
- * Returns true when the candidate can produce a polygon.
+/* Early rejection, evaluated on primal values before the full clip: it
+ * returns false only for candidates the clip leaves without a polygon, so
+ * it saves work and never changes a result (the clip re-tests whatever is
+ * kept).
  * soft-rigid:
- *  - the gradient filter of process_candidate (same arithmetic);
  *  - all three triangle vertices farther than 1e-15 outside one tet face
  *    plane: every vertex the clip creates is an affine combination of
- *    them (rounding ~1e-17), so that stage leaves nothing;
- *  - the tet's AABB farther than 1e-12 from the triangle's plane: the
- *    clipped polygon stays within rounding of that plane and cannot meet
- *    the tet.
+ *    them (rounding ~1e-17), so that stage leaves nothing (tested on the
+ *    float copy of the planes when the mesh has one);
+ *  - the gradient filter of process_candidate (same arithmetic);
+ *  - (before queueing, box_may_meet_triangle) the tet's AABB farther than
+ *    1e-12 from the triangle's plane or beyond an in-plane edge axis.
  * soft-soft: the equal-pressure plane, both gradient filters and the
  * marching-tets sign test of process_candidate (same arithmetic). */
 template <int KS = -1>
@@ -515,29 +517,47 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
   if (kind != KIND_SOFT) {
     /* tv, tn: triangle b's vertices and unit normal already moved into A's
      * frame (xform / rot with X, the same arithmetic as process_candidate) */
-    const double nS[3] = {tn[0], tn[1], tn[2]};
-    if (!(((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA)) return false;
     double v[3][3];
+    double vmax = 0.0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) v[i][0] = tv[3 * i], v[i][1] = tv[3 * i + 1], v[i][2] = tv[3 * i + 2];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const double* n = r + 3 * k;
-      const double d = r[12 + k];
-      bool out = true;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) out = out && (((n[0] * v[i][0] + n[1] * v[i][1]) + n[2] * v[i][2]) + d > 1e-15);
-      if (out) return false;
+    for (int i = 0; i < 3; ++i) {
+      v[i][0] = tv[3 * i], v[i][1] = tv[3 * i + 1], v[i][2] = tv[3 * i + 2];
+      vmax = fmax(vmax, fmax(fabs(v[i][0]), fmax(fabs(v[i][1]), fabs(v[i][2]))));
     }
-    const double* bx = E.boxA + 6 * (size_t)a;
-    double dist = -((nS[0] * v[0][0] + nS[1] * v[0][1]) + nS[2] * v[0][2]);
-    double rad = 0.0;
+    if (E.fplA) {
+      /* the face-plane test on the float planes (64 B instead of 128 B of
+       * the record): each float differs from its double by at most 2^-24
+       * of it, so n.v + d moves by at most 2^-24 (3 vmax + |d|) (unit n);
+       * a vertex more than twice that (plus 1e-15) outside is outside in
+       * the double test too, so every rejection here is one the double
+       * test makes.  The double test is not repeated for the rest: it only
+       * saves clip work, and the clip decides exactly whatever it keeps. */
+      const float4* fp = E.fplA + 4 * (size_t)a;
+      float4 pl[4];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      dist += nS[c] * (0.5 * (bx[c] + bx[3 + c]));
-      rad += fabs(nS[c]) * (0.5 * (bx[3 + c] - bx[c]));
+      for (int k = 0; k < 4; ++k) pl[k] = fp[k];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double n0 = pl[k].x, n1 = pl[k].y, n2 = pl[k].z, d = pl[k].w;
+        const double tol = 1.2e-7 * (3.0 * vmax + fabs(d)) + 1e-15;
+        bool out = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out = out && (((n0 * v[i][0] + n1 * v[i][1]) + n2 * v[i][2]) + d > tol);
+        if (out) return false;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const double* n = r + 3 * k;
+        const double d = r[12 + k];
+        bool out = true;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out = out && (((n[0] * v[i][0] + n[1] * v[i][1]) + n[2] * v[i][2]) + d > 1e-15);
+        if (out) return false;
+      }
     }
-    return !(fabs(dist) > rad + 1e-12);
+    const double nS[3] = {tn[0], tn[1], tn[2]};
+    return ((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA;
   }
   const double* rb = E.recB + (size_t)HCS_TREC * b;
   const double sc = 1.0;
@@ -627,16 +647,17 @@ __device__ inline bool box_may_meet_plane(const double* X, const double* Q) {
  * HCS_STACK - HCS_LSTACK entries.  DFS over these LBVHs keeps well under 16
  * nodes pending, so the overflow is not touched in practice. */
 #ifndef HCS_LSTACK
-#define HCS_LSTACK 16
+#define HCS_LSTACK 12
 #endif
 
-/* 13.1 KB, so 12 waves fit in a CU's 160 KB of LDS.  Query boxes are not
+/* 13.1 KB, so 12 waves fit in a CU's 160 KB of LDS.  q1 takes up to 63
+ * queued candidates plus 4 leaf hits from each of 64 lanes.  Query boxes are not
  * stored: a lane keeps its own in registers, and a box is rebuilt from the
  * vertices in tv / tn (min / max, exact) when a lane steals a subtree or a
  * tile triangle is tested. */
 struct CandLds {
   int stk[HCS_LSTACK * 64];
-  int2 q1[192];
+  int2 q1[320];
   int2 q2[128]; /* while producing, q2 holds fewer than 64 entries and
                    q2[64..127] carries the work-stealing hand-over */
   double tv[64 * 9]; /* query / tile triangles (first three vertices of a
@@ -781,7 +802,10 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
     if (E.ntA == 1 || item.root < 0) {
       /* a one-tet mesh, or a sub-item whose subtree is a single leaf */
       const int leaf = E.ntA == 1 ? 0 : ~item.root;
-      const bool hit = valid && box_overlap(E.boxA + 6 * (size_t)leaf, lo, hi);
+      const double* lb = E.boxA + 6 * (size_t)leaf;
+      bool hit = valid && box_overlap(lb, lo, hi);
+      if (hit && MODE == CAND_STEP && kind == KIND_TRI && !box_may_meet_triangle(lb, &L.tv[9 * lane], &L.tn[3 * lane]))
+        hit = false, ++nc;
       const unsigned long long m = __ballot(hit);
       if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(leaf, b);
       q1n += __popcll(m);
@@ -846,7 +870,8 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
           tmask &= tmask - 1;
           double tl[3], th[3];
           slot_box(L, jj, false, tl, th);
-          const bool hit = valid && box_overlap(bx, tl, th);
+          bool hit = valid && box_overlap(bx, tl, th);
+          if (hit && MODE == CAND_STEP && !box_may_meet_triangle(bx, &L.tv[9 * jj], &L.tn[3 * jj])) hit = false, ++nc;
           const unsigned long long m = __ballot(hit);
           if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, t0 + jj);
           q1n += __popcll(m);
@@ -886,9 +911,10 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
             }
             __syncthreads();
           }
-          /* one traversal step (a tree of ntA leaves has ntA-1 internal
-           * nodes; the visit cap only guards against a malformed tree) */
-          int c0 = -1, c1 = -1;
+          /* one step over a wide node (a tree of ntA leaves has ntA-1
+           * internal nodes; the visit cap only guards against a malformed
+           * tree) */
+          int lh[4] = {-1, -1, -1, -1};
           if (cnt > 0 && ++visits > visit_cap) {
             atomicOr(A.error, 2);
             bp = sp = on = 0;
@@ -896,20 +922,25 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
           }
           if (cnt > 0) {
             const int nd = sp > bp ? L.stk[((--sp) % HCS_LSTACK) * 64 + lane] : ovf[--on];
-            const Node* N = E.nodesA + nd;
-            double bb[12];
+            const WNode* N = E.wnodesA + nd;
+            double bb[24];
 #pragma unroll
-            for (int i = 0; i < 12; ++i) bb[i] = N->b[i];
-            const int ch0 = N->c[0], ch1 = N->c[1];
-            const bool o0 = box_overlap(bb, lo, hi), o1 = box_overlap(bb + 6, lo, hi);
-            /* push: a full LDS window first spills its bottom entry */
+            for (int i = 0; i < 24; ++i) bb[i] = N->b[i];
+            const int4 ch = *(const int4*)N->ch;
+            const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
+            /* push in reverse, so child 0 is walked first; a full LDS
+             * window first spills its bottom entry */
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-              const bool o = k == 0 ? o1 : o0;
-              const int ch = k == 0 ? ch1 : ch0;
-              if (!o) continue;
-              if (ch < 0) {
-                (k == 0 ? c1 : c0) = ~ch;
+            for (int k = 3; k >= 0; --k) {
+              const int c = cc[k];
+              if (c == HCS_WNONE || !box_overlap(bb + 6 * k, lo, hi)) continue;
+              if (c < 0) {
+                lh[k] = ~c;
+                /* a leaf whose tet box cannot meet the triangle: a
+                 * candidate the clip leaves empty, counted, not queued */
+                if (MODE == CAND_STEP && kind == KIND_TRI &&
+                    !box_may_meet_triangle(bb + 6 * k, &L.tv[9 * b], &L.tn[3 * b]))
+                  lh[k] = -1, ++nc;
                 continue;
               }
               if (sp - bp == HCS_LSTACK) {
@@ -920,26 +951,21 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
                 ovf[on++] = L.stk[(bp % HCS_LSTACK) * 64 + lane];
                 ++bp;
               }
-              L.stk[(sp % HCS_LSTACK) * 64 + lane] = ch;
+              L.stk[(sp % HCS_LSTACK) * 64 + lane] = c;
               ++sp;
             }
-            if (MODE == CAND_STEP && kind == KIND_TRI) {
-              /* leaf hits whose tet box cannot meet the triangle are
-               * candidates that the clip would leave empty: counted, not
-               * queued */
-              if (c0 >= 0 && !box_may_meet_triangle(bb, &L.tv[9 * b], &L.tn[3 * b])) c0 = -1, ++nc;
-              if (c1 >= 0 && !box_may_meet_triangle(bb + 6, &L.tv[9 * b], &L.tn[3 * b])) c1 = -1, ++nc;
-            }
           }
-          const unsigned long long m0 = __ballot(c0 >= 0), m1 = __ballot(c1 >= 0);
-          if (c0 >= 0) L.q1[q1n + lane_prefix(m0)] = make_int2(c0, b);
-          if (c1 >= 0) L.q1[q1n + __popcll(m0) + lane_prefix(m1)] = make_int2(c1, b);
-          q1n += __popcll(m0) + __popcll(m1);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const unsigned long long m = __ballot(lh[k] >= 0);
+            if (lh[k] >= 0) L.q1[q1n + lane_prefix(m)] = make_int2(lh[k], b);
+            q1n += __popcll(m);
+          }
         }
       }
     }
     /* ---- consume: cheap tests on 64 candidates, flush 64 survivors;
-     * q1 < 192 and q2 < 128 hold throughout */
+     * q1 < 320 and q2 < 128 hold throughout */
     for (;;) {
       const bool do2 = q2n >= 64 || (!producing && q1n == 0 && q2n > 0);
       const bool do1 = q1n >= 64 || ((!producing || drain) && q1n > 0);

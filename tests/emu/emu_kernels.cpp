@@ -42,7 +42,9 @@ int delta(const std::vector<unsigned>& c, int n, int i, int j) {
 
 struct Soft {
   std::vector<double> rec, box, boxL;
+  std::vector<float4> fpl;
   std::vector<Node> nodes;
+  std::vector<WNode> wnodes;
   std::vector<int> leaf;
 };
 
@@ -52,8 +54,12 @@ void prepare(const double* V, int nv, const int32_t* T, int nt, const double* p,
   orc_tet_precompute(V, nv, T, nt, p, r32.data());
   s.rec.assign(24 * (size_t)nt, 0.0);
   s.box.resize(6 * (size_t)nt);
+  s.fpl.resize(4 * (size_t)nt);
   for (int t = 0; t < nt; ++t) {
     for (int k = 0; k < 23; ++k) s.rec[24 * (size_t)t + k] = r32[32 * (size_t)t + k];
+    const double* r = &s.rec[24 * (size_t)t];
+    for (int k = 0; k < 4; ++k)
+      s.fpl[4 * (size_t)t + k] = make_float4((float)r[3 * k], (float)r[3 * k + 1], (float)r[3 * k + 2], (float)r[12 + k]);
     for (int k = 0; k < 6; ++k) s.box[6 * (size_t)t + k] = r32[32 * (size_t)t + 23 + k];
   }
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -119,6 +125,8 @@ void prepare(const double* V, int nv, const int32_t* T, int nt, const double* p,
       s.nodes[pp].h[side] = h;
     }
   }
+  s.wnodes.resize(std::max(1, n - 1));
+  for (int i = 0; i + 1 < n; ++i) make_wide_node(s.nodes.data(), i, s.wnodes[i]);
 }
 
 }  // namespace
@@ -153,7 +161,7 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
     orc_tri_normals(VB, nvb, FB, nb, nrm.data());
   }
   EnvDesc E{};
-  E.recA = A.rec.data(), E.boxA = A.box.data(), E.nodesA = A.nodes.data(), E.leafA = A.leaf.data();
+  E.recA = A.rec.data(), E.fplA = A.fpl.data(), E.boxA = A.box.data(), E.nodesA = A.nodes.data(), E.wnodesA = A.wnodes.data(), E.leafA = A.leaf.data();
   E.boxLA = A.boxL.data(), E.TA = TA, E.VA = VA, E.ntA = nta;
   E.VB = VB, E.FB = FB, E.nrmB = nrm.empty() ? nullptr : nrm.data();
   E.recB = kind == KIND_SOFT ? Bs.rec.data() : nullptr;

@@ -25,6 +25,10 @@ struct int4 {
   int x, y, z, w;
 };
 inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
+struct float4 {
+  float x, y, z, w;
+};
+inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
 inline int min(int a, int b) { return a < b ? a : b; }
 
 struct Dim3 {
