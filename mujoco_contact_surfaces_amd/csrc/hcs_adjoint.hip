@@ -61,6 +61,7 @@ static void seeds(Ctx* c, const hcs_pair_in* in, int npairs, std::vector<double>
 static int tangent_pass(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, int dir0, int ndir,
                         double* poly_rec, double* poly_tan, int64_t* poly_key, int64_t poly_cap, Plan& plan,
                         unsigned long long* npoly) {
+  if (ndir < 1 || ndir > HCS_MAXDIR) return set_err(c, HCS_ERR_ARG, "tangent directions out of range");
   int rc = stage_pairs(c, npairs, in, opts, plan);
   if (rc) return rc;
   const int nitems = (int)plan.items.size();

@@ -551,41 +551,79 @@ __global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int di
     clip_survivor<T, KS>(A, C, i, dir, dst, op);
 }
 
-/* Per pair (one wave, grid.y = direction): its survivors' contributions
- * summed lane-strided in order, then a fixed butterfly; candidates summed
- * over its items. */
-__global__ __launch_bounds__(64) void k_env_reduce(const EnvDesc* __restrict__ envs, const unsigned* __restrict__ off,
+/* Per pair, in K_e waves: wave w = (pair e, slice j, K_e, first partial)
+ * from the plan's table sums slice j of the pair's contributions (a fixed
+ * split of its slot range) lane-strided in order, then a fixed butterfly,
+ * and the slice's items' candidate counts likewise; grid.y = direction.
+ * k_env_combine adds a pair's slices in order, so results do not depend on
+ * timing, and a heavy pair (K_e grows with its item count) no longer sets
+ * the time of the whole reduction. */
+struct RedPart {
+  double F[3], tA[3], tB[3];
+  int64_t np, nf, nc;
+};
+
+__global__ __launch_bounds__(64) void k_env_reduce(const EnvDesc* __restrict__ envs, const int4* __restrict__ rtab,
+                                                   int nred, const unsigned* __restrict__ off,
                                                    const int* __restrict__ item_ncand,
                                                    const unsigned long long* __restrict__ surv_cnt, int64_t surv_cap,
                                                    const Contrib* __restrict__ contrib, int64_t stride,
-                                                   const int* __restrict__ err, PairOut* __restrict__ out, int n) {
-  const int e = blockIdx.x, d = blockIdx.y, lane = threadIdx.x;
+                                                   RedPart* __restrict__ part) {
+  const int4 w = rtab[blockIdx.x];
+  const int e = w.x, j = w.y, K = w.z, d = blockIdx.y, lane = threadIdx.x;
   const int first = envs[e].item_first, cnt = envs[e].item_count;
   /* on survivor-list overflow the offsets run past the contribution slots:
-   * sum nothing, the status below reports the overflow */
+   * sum nothing, the status reports the overflow */
   const bool over = (int64_t)*surv_cnt > surv_cap;
   const unsigned lo = off[first], hi = over ? lo : off[first + cnt];
+  const uint64_t len = hi - lo;
+  const unsigned s0 = lo + (unsigned)(len * j / K), s1 = lo + (unsigned)(len * (j + 1) / K);
+  const int i0 = first + (int)((int64_t)cnt * j / K), i1 = first + (int)((int64_t)cnt * (j + 1) / K);
   const Contrib* cb = contrib + (int64_t)d * stride;
   double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
   int64_t np = 0, nf = 0, nc = 0;
-  for (unsigned i = lo + lane; i < hi; i += 64) {
+  for (unsigned i = s0 + lane; i < s1; i += 64) {
     const Contrib& q = cb[i];
 #pragma unroll
     for (int c = 0; c < 3; ++c) F[c] = F[c] + q.F[c], tA[c] = tA[c] + q.tA[c], tB[c] = tB[c] + q.tB[c];
     np += q.np, nf += q.nf;
   }
-  for (int i = first + lane; i < first + cnt; i += 64) nc += item_ncand[i];
+  for (int i = i0 + lane; i < i1; i += 64) nc += item_ncand[i];
 #pragma unroll
   for (int c = 0; c < 3; ++c) F[c] = wave_sum(F[c]), tA[c] = wave_sum(tA[c]), tB[c] = wave_sum(tB[c]);
   np = wave_sum_i(np), nf = wave_sum_i(nf), nc = wave_sum_i(nc);
   if (lane == 0) {
-    PairOut r;
+    RedPart r;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) r.F_A[c] = F[c], r.tau_A[c] = tA[c], r.F_B[c] = -F[c], r.tau_B[c] = tB[c];
-    r.n_candidates = nc, r.n_polygons = np, r.n_faces = nf;
-    r.status = (err && *err) ? HCS_ERR_LIMIT : (over ? HCS_ERR_CAPACITY : 0);
-    out[(int64_t)d * n + e] = r;
+    for (int c = 0; c < 3; ++c) r.F[c] = F[c], r.tA[c] = tA[c], r.tB[c] = tB[c];
+    r.np = np, r.nf = nf, r.nc = nc;
+    part[(int64_t)d * nred + w.w + j] = r;
   }
+}
+
+/* One thread per (pair, direction): the pair's slices in order. */
+__global__ void k_env_combine(const int2* __restrict__ ptab, int nred, const RedPart* __restrict__ part,
+                              const unsigned long long* __restrict__ surv_cnt, int64_t surv_cap,
+                              const int* __restrict__ err, PairOut* __restrict__ out, int n, int ndir) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * ndir) return;
+  const int e = (int)(t % n), d = (int)(t / n);
+  const int2 pe = ptab[e];
+  const RedPart* p = part + (int64_t)d * nred + pe.x;
+  double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
+  int64_t np = 0, nf = 0, nc = 0;
+  for (int j = 0; j < pe.y; ++j) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) F[c] = F[c] + p[j].F[c], tA[c] = tA[c] + p[j].tA[c], tB[c] = tB[c] + p[j].tB[c];
+    np += p[j].np, nf += p[j].nf, nc += p[j].nc;
+  }
+  const bool over = (int64_t)*surv_cnt > surv_cap;
+  PairOut r;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r.F_A[c] = F[c], r.tau_A[c] = tA[c], r.F_B[c] = -F[c], r.tau_B[c] = tB[c];
+  r.n_candidates = nc, r.n_polygons = np, r.n_faces = nf;
+  r.status = (err && *err) ? HCS_ERR_LIMIT : (over ? HCS_ERR_CAPACITY : 0);
+  out[t] = r;
 }
 
 void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double* xf, int n) {
@@ -711,11 +749,15 @@ static void pipe_back(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_clip<double, -1>), dim3(g), dim3(256), 0, s, a, C, -1,
                        (Contrib*)p.contrib.ptr, (int64_t)0);
   if (e_reduce) (void)hipEventRecord(e_reduce, s);
-  if (npairs > 0)
-    hipLaunchKernelGGL(k_env_reduce, dim3(npairs, std::max(1, ndir)), dim3(64), 0, s, a.envs,
+  if (npairs > 0 && p.nred > 0) {
+    const int nd = std::max(1, ndir);
+    hipLaunchKernelGGL(k_env_reduce, dim3(p.nred, nd), dim3(64), 0, s, a.envs, (const int4*)p.rtab.ptr, p.nred,
                        (const unsigned*)p.off.ptr, (const int*)p.ncand.ptr, (const unsigned long long*)p.surv_cnt.ptr,
-                       p.cap, (const Contrib*)p.contrib.ptr, (int64_t)p.cap, (const int*)p.err.ptr,
-                       (PairOut*)p.out.ptr, npairs);
+                       p.cap, (const Contrib*)p.contrib.ptr, (int64_t)p.cap, (RedPart*)p.part.ptr);
+    hipLaunchKernelGGL(k_env_combine, dim3((npairs * nd + 127) / 128), dim3(128), 0, s, (const int2*)p.ptab.ptr,
+                       p.nred, (const RedPart*)p.part.ptr, (const unsigned long long*)p.surv_cnt.ptr, p.cap,
+                       (const int*)p.err.ptr, (PairOut*)p.out.ptr, npairs, nd);
+  }
 }
 
 void pipe_enqueue(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cand_grid, int clip_grid, int ndir,
@@ -751,6 +793,31 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
   p.nxenv = (int)plan.xenv.size();
   p.kinds = 0;
   for (const EnvDesc& E : plan.envs) p.kinds |= 1 << E.kind;
+  /* reduction slices: a pair of n items gets ceil(n / 8) waves (at most
+   * 64); a split heavy pair has many items */
+  {
+    std::vector<int4> rt;
+    std::vector<int2> pt(std::max<size_t>(1, plan.envs.size()));
+    for (size_t e = 0; e < plan.envs.size(); ++e) {
+      const int K = std::max(1, std::min(64, (plan.envs[e].item_count + 7) / 8));
+      pt[e] = make_int2((int)rt.size(), K);
+      for (int j = 0; j < K; ++j) rt.push_back(make_int4((int)e, j, K, (int)rt.size() - j));
+    }
+    p.nred = (int)rt.size();
+    if ((rc = ensure(c, p.rtab, sizeof(int4) * std::max<size_t>(1, rt.size()))) ||
+        (rc = ensure(c, p.ptab, sizeof(int2) * pt.size())) ||
+        (rc = ensure(c, p.part, sizeof(RedPart) * HCS_MAXDIR * std::max<size_t>(1, rt.size()))))
+      return rc;
+    p.rtab_host = rt;
+    p.ptab_host = pt;
+    if (!rt.empty())
+      rc = hip_check(c, hipMemcpyAsync(p.rtab.ptr, p.rtab_host.data(), sizeof(int4) * rt.size(), hipMemcpyHostToDevice,
+                                       c->stream), "upload reduction slices");
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(p.ptab.ptr, p.ptab_host.data(), sizeof(int2) * pt.size(),
+                                       hipMemcpyHostToDevice, c->stream), "upload reduction slices");
+    if (rc) return rc;
+  }
   p.ordered = (int)plan.order.size() == nitems && nitems > 0;
   if (p.ordered) {
     if ((rc = ensure(c, p.order, sizeof(int) * ni))) return rc;
@@ -773,7 +840,8 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
 
 void pipe_free(Pipe& p) {
   for (DBuf* b : {&p.counter, &p.surv, &p.surv_cnt, &p.ncand, &p.nsurv, &p.off, &p.scan_tmp, &p.contrib, &p.out, &p.err,
-                  &p.xf, &p.ibox, &p.live, &p.live_item, &p.qseg, &p.xtri, &p.xenv, &p.order})
+                  &p.xf, &p.ibox, &p.live, &p.live_item, &p.qseg, &p.xtri, &p.xenv, &p.order, &p.part, &p.rtab,
+                  &p.ptab})
     if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
 }
 

@@ -42,14 +42,18 @@ struct DBuf {
  * per-item survivor counts, k_clip, k_env_reduce. */
 struct Pipe {
   DBuf counter /* [0..7] work counters, [8..15] live items per queue */, surv, surv_cnt, ncand, nsurv, off, scan_tmp,
-      contrib, out, err, xf, ibox, live, live_item, qseg, xtri, xenv, order;
+      contrib, out, err, xf, ibox, live, live_item, qseg, xtri, xenv, order, part /* per-slice sums */,
+      rtab /* reduction waves (pair, slice, slices, first partial) */, ptab /* per pair (first partial, slices) */;
   int64_t cap = 0;
   size_t scan_bytes = 0;
   std::vector<int> qseg_host; /* host copy of the queue segments (the upload source) */
   int nqueues = 1;            /* candidate-kernel work queues (1..HCS_QUEUES) */
   int nxenv = 0;              /* pairs in tet-query mode (their triangles are moved per step) */
   bool ordered = false;       /* order holds a work order of the items */
-  int kinds = 0;              /* bit k set: some pair of the plan is of EnvKind k (0: not known) */
+  int kinds = 0;
+  int nred = 0;                 /* reduction waves */
+  std::vector<int4> rtab_host;  /* upload sources of rtab / ptab */
+  std::vector<int2> ptab_host;              /* bit k set: some pair of the plan is of EnvKind k (0: not known) */
 };
 
 struct Ctx;

@@ -33,6 +33,8 @@ namespace hcs {
 #define HCS_FREC 12
 #define HCS_PREC 40
 #define HCS_STATE 36
+/* tangent directions of the differentiation pass: 6 of the pose of B, E_A */
+#define HCS_MAXDIR 7
 /* a rigid triangle moved into A's frame: 3 vertices, unit normal, AABB */
 #define HCS_XTRI 18
 /* at most this many work queues for the candidate kernel: with one per XCD

@@ -524,6 +524,11 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
       v[i][0] = tv[3 * i], v[i][1] = tv[3 * i + 1], v[i][2] = tv[3 * i + 2];
       vmax = fmax(vmax, fmax(fabs(v[i][0]), fmax(fabs(v[i][1]), fabs(v[i][2]))));
     }
+    /* the gradient filter, computed before the plane test so that both
+     * loads are in flight together */
+    const double nS[3] = {tn[0], tn[1], tn[2]};
+    const bool filt = ((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA;
+    bool sep = false;
     if (E.fplA) {
       /* the face-plane test on the float planes (64 B instead of 128 B of
        * the record): each float differs from its double by at most 2^-24
@@ -543,7 +548,7 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
         bool out = true;
 #pragma unroll
         for (int i = 0; i < 3; ++i) out = out && (((n0 * v[i][0] + n1 * v[i][1]) + n2 * v[i][2]) + d > tol);
-        if (out) return false;
+        sep = sep || out;
       }
     } else {
 #pragma unroll
@@ -553,11 +558,10 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
         bool out = true;
 #pragma unroll
         for (int i = 0; i < 3; ++i) out = out && (((n[0] * v[i][0] + n[1] * v[i][1]) + n[2] * v[i][2]) + d > 1e-15);
-        if (out) return false;
+        sep = sep || out;
       }
     }
-    const double nS[3] = {tn[0], tn[1], tn[2]};
-    return ((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA;
+    return filt && !sep;
   }
   const double* rb = E.recB + (size_t)HCS_TREC * b;
   const double sc = 1.0;
