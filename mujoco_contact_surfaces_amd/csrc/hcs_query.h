@@ -440,7 +440,8 @@ __device__ inline bool box_may_meet_triangle(const double* bx, const double* v, 
     double bmin = sg * dot3(m, c);
 #pragma unroll
     for (int k = 0; k < 3; ++k) bmin -= fabs(m[k]) * h[k];
-    if (bmin - tmax > 1e-12 * sqrt(dot3(m, m))) return false;
+    const double gap = bmin - tmax;
+    if (gap > 0.0 && gap * gap > 1e-24 * dot3(m, m)) return false;
   }
   return true;
 }
