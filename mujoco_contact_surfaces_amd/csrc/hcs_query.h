@@ -412,19 +412,30 @@ __device__ inline bool box_overlap(const double* a /*lo,hi*/, const double* lo, 
  * test as the last of survives()) and, per edge, the in-plane direction
  * m = edge x n turned away from the opposite vertex, against the
  * triangle's largest projection on m.  A box that one axis separates from
- * the triangle by more than 1e-12 of the axis length holds no point within
- * rounding of the triangle, so clipping the triangle by the tet leaves
- * nothing; the axis itself may be rounded, since any axis separates. */
+ * the triangle by more than the margin holds no point within rounding of
+ * the triangle, so clipping the triangle by the tet leaves nothing; the
+ * axis itself may be rounded, since any axis separates.  The margin is
+ * 1e-12 of the axis length plus 32 ulps of the coordinates the projections
+ * add up (sum_k |m_k| (|c_k| + h_k + vmax)), so it stays above the rounding
+ * of the projections and of the clip for meshes stored far from their
+ * frame's origin or in large units. */
+#define HCS_ULP32 (32.0 * 2.220446049250313e-16)
 __device__ inline bool box_may_meet_triangle(const double* bx, const double* v, const double* n) {
   double c[3], h[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) c[k] = 0.5 * (bx[k] + bx[3 + k]), h[k] = 0.5 * (bx[3 + k] - bx[k]);
+  double vmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) vmax = fmax(vmax, fabs(v[i]));
+  double mag[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) mag[k] = (fabs(c[k]) + h[k]) + vmax;
   {
     double dist = -((n[0] * v[0] + n[1] * v[1]) + n[2] * v[2]);
-    double rad = 0.0;
+    double rad = 0.0, sc = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) dist += n[k] * c[k], rad += fabs(n[k]) * h[k];
-    if (fabs(dist) > rad + 1e-12) return false;
+    for (int k = 0; k < 3; ++k) dist += n[k] * c[k], rad += fabs(n[k]) * h[k], sc += fabs(n[k]) * mag[k];
+    if (fabs(dist) > rad + (1e-12 + HCS_ULP32 * sc)) return false;
   }
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
@@ -437,10 +448,11 @@ __device__ inline bool box_may_meet_triangle(const double* bx, const double* v, 
     const double pa = dot3(m, a), pb = dot3(m, b), po = dot3(m, o);
     const double sg = po > pa ? -1.0 : 1.0;
     const double tmax = sg > 0.0 ? fmax(fmax(pa, pb), po) : fmax(fmax(-pa, -pb), -po);
-    double bmin = sg * dot3(m, c);
+    double bmin = sg * dot3(m, c), sc = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) bmin -= fabs(m[k]) * h[k];
-    const double gap = bmin - tmax;
+    for (int k = 0; k < 3; ++k) bmin -= fabs(m[k]) * h[k], sc += fabs(m[k]) * mag[k];
+    /* gap beyond the rounding margin, then beyond 1e-12 |m| (no square root) */
+    const double gap = (bmin - tmax) - HCS_ULP32 * sc;
     if (gap > 0.0 && gap * gap > 1e-24 * dot3(m, m)) return false;
   }
   return true;
@@ -489,13 +501,15 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
  * it saves work and never changes a result (the clip re-tests whatever is
  * kept).
  * soft-rigid:
- *  - all three triangle vertices farther than 1e-15 outside one tet face
- *    plane: every vertex the clip creates is an affine combination of
- *    them (rounding ~1e-17), so that stage leaves nothing (tested on the
- *    float copy of the planes when the mesh has one);
+ *  - all three triangle vertices farther than 1e-15 + 32 ulps of
+ *    3 max|v| + |d| outside one tet face plane n.x + d: every vertex the
+ *    clip creates is a convex combination of them, rounded by a few ulps
+ *    of those magnitudes at each of the four stages, so that stage leaves
+ *    nothing (tested on the float copy of the planes when the mesh has
+ *    one);
  *  - the gradient filter of process_candidate (same arithmetic);
- *  - (before queueing, box_may_meet_triangle) the tet's AABB farther than
- *    1e-12 from the triangle's plane or beyond an in-plane edge axis.
+ *  - (before queueing, box_may_meet_triangle) the tet's AABB beyond the
+ *    triangle's plane or an in-plane edge axis by more than its margin.
  * soft-soft: the equal-pressure plane, both gradient filters and the
  * marching-tets sign test of process_candidate (same arithmetic). */
 template <int KS = -1>
@@ -534,8 +548,8 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
       /* the face-plane test on the float planes (64 B instead of 128 B of
        * the record): each float differs from its double by at most 2^-24
        * of it, so n.v + d moves by at most 2^-24 (3 vmax + |d|) (unit n);
-       * a vertex more than twice that (plus 1e-15) outside is outside in
-       * the double test too, so every rejection here is one the double
+       * a vertex more than twice that (plus the double test's own margin)
+       * outside is outside in the double test too, so every rejection here is one the double
        * test makes.  The double test is not repeated for the rest: it only
        * saves clip work, and the clip decides exactly whatever it keeps. */
       const float4* fp = E.fplA + 4 * (size_t)a;
@@ -545,7 +559,7 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double n0 = pl[k].x, n1 = pl[k].y, n2 = pl[k].z, d = pl[k].w;
-        const double tol = 1.2e-7 * (3.0 * vmax + fabs(d)) + 1e-15;
+        const double tol = (1.2e-7 + HCS_ULP32) * (3.0 * vmax + fabs(d)) + 1e-15;
         bool out = true;
 #pragma unroll
         for (int i = 0; i < 3; ++i) out = out && (((n0 * v[i][0] + n1 * v[i][1]) + n2 * v[i][2]) + d > tol);
@@ -556,9 +570,10 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
       for (int k = 0; k < 4; ++k) {
         const double* n = r + 3 * k;
         const double d = r[12 + k];
+        const double tol = HCS_ULP32 * (3.0 * vmax + fabs(d)) + 1e-15;
         bool out = true;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) out = out && (((n[0] * v[i][0] + n[1] * v[i][1]) + n[2] * v[i][2]) + d > 1e-15);
+        for (int i = 0; i < 3; ++i) out = out && (((n[0] * v[i][0] + n[1] * v[i][1]) + n[2] * v[i][2]) + d > tol);
         sep = sep || out;
       }
     }

@@ -295,3 +295,13 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
 }
 
 }  // extern "C"
+
+extern "C" {
+
+/* The separating-axis prefilter of the candidate kernel on one box
+ * (lo[3], hi[3]) and one triangle (3 x 3 vertices, unit normal n). */
+int emu_box_may_meet_triangle(const double* box, const double* tri, const double* n) {
+  return box_may_meet_triangle(box, tri, n) ? 1 : 0;
+}
+
+}  // extern "C"

@@ -39,6 +39,8 @@ def emu(tmp_path_factory):
     lib.emu_pair.restype = C.c_int
     lib.emu_pair.argtypes = [_D, C.c_int, _I32, C.c_int, _D, _D, C.c_int, _I32, C.c_int, _D, C.c_int, _D, _D,
                              C.c_int, C.c_int, _D, _D, _I64, _I64]
+    lib.emu_box_may_meet_triangle.restype = C.c_int
+    lib.emu_box_may_meet_triangle.argtypes = [_D, _D, _D]
     return lib
 
 
@@ -190,3 +192,61 @@ def test_tangent_pass_matches_central_differences(emu, kind):
         hE = 1e-7 * E
         fd = (oracle(sc, kind, E_A=E + hE)["wrench"] - oracle(sc, kind, E_A=E - hE)["wrench"]) / (2 * hE)
         assert np.abs(W[7] - fd).max() <= 1e-6 * np.abs(W[7]).max() + 1e-12 * np.abs(W[0]).max() / E
+
+
+def _far_from_origin(sc, off):
+    """The same scene with both meshes stored `off` away from their frames'
+    origins (coordinates of magnitude |off| in every frame; world geometry
+    unchanged up to rounding)."""
+    sc = dict(sc)
+    s = dict(sc["soft"])
+    r = dict(sc["rigid"])
+    oa, ob = np.asarray(off[0], float), np.asarray(off[1], float)
+    s["V"] = s["V"] + oa
+    r["V"] = r["V"] + ob
+    XA, XB = sc["X_WA"].copy(), sc["X_WB"].copy()
+    XA[9:] = XA[9:] - XA[:9].reshape(3, 3) @ oa
+    XB[9:] = XB[9:] - XB[:9].reshape(3, 3) @ ob
+    sc.update(soft=s, rigid=r, X_WA=XA, X_WB=XB)
+    return sc
+
+
+@pytest.mark.parametrize("off", [3e3, 3e4])
+@pytest.mark.parametrize("kind", [KTRI, KTET])
+def test_prefilters_hold_for_meshes_far_from_their_origin(emu, kind, off):
+    """The separating-axis and face-plane prefilters only drop candidates
+    that the clip leaves empty, also when coordinates are large (their
+    margins scale with the coordinates involved)."""
+    sc = _far_from_origin(_scene(kind), ([off, -0.7 * off, 0.4 * off], [-0.3 * off, off, 0.9 * off]))
+    W, counts, stats = run_pair(emu, sc, kind)
+    ref = oracle(sc, kind)
+    assert ref["counts"][1] > 0
+    np.testing.assert_array_equal(counts, ref["counts"])
+    assert_wrench_close(W[0], ref["wrench"])
+    assert stats[4] == 0
+
+
+@pytest.mark.parametrize("scale", [1.0, 3e3, 3e4, 1e6])
+def test_separating_axis_prefilter_keeps_touching_boxes(emu, scale):
+    """A box that contains a point of the triangle (here: a vertex, or the
+    midpoint of an edge, on the box's boundary) must never be reported as
+    separated, whatever the magnitude of the coordinates."""
+    rng = np.random.default_rng(int(scale) % 1000003)
+    bad = 0
+    for _ in range(400):
+        base = rng.uniform(-1, 1, 3) * scale
+        tri = base + rng.uniform(-1, 1, (3, 3)) * 0.05
+        e1, e2 = tri[1] - tri[0], tri[2] - tri[0]
+        n = np.cross(e1, e2)
+        n = n / np.linalg.norm(n)
+        k = rng.integers(0, 3)
+        P = tri[k] if rng.integers(0, 2) else 0.5 * (tri[k] + tri[(k + 1) % 3])
+        # P on one face of the box, or at a corner; the box extends away
+        size = rng.uniform(1e-4, 1e-2, 3)
+        side = rng.integers(0, 3, 3)  # 0: P on lo, 1: P on hi, 2: P inside
+        lo = np.where(side == 0, P, np.where(side == 1, P - size, P - 0.5 * size))
+        hi = np.where(side == 0, P + size, np.where(side == 1, P, P + 0.5 * size))
+        box = _f(np.concatenate([lo, hi]))
+        assert np.all(box[:3] <= P) and np.all(P <= box[3:])
+        bad += emu.emu_box_may_meet_triangle(_dp(box), _dp(_f(tri)), _dp(_f(n))) == 0
+    assert bad == 0
