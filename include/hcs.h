@@ -178,6 +178,17 @@ int hcs_add_rigid_mesh(hcs_ctx* ctx, const double* V, int32_t nv, const int32_t*
  * ComputeContactSurfaceFromSoftVolumeRigidHalfSpace).  Pairs (soft, plane)
  * slice the soft tets by the plane z = 0; face keys are (pair, tet, 0, k). */
 int hcs_add_rigid_plane(hcs_ctx* ctx, const hcs_props* props, int32_t geom_id, int32_t* handle);
+/* onGeomChanged (plugin.cpp:828-975): new geometry for an existing soft or
+ * rigid mesh handle (the geom was resized or re-meshed), keeping its cs::
+ * properties and geom_id.  Unlike the reference, which leaves the new
+ * pressure field out of the geometry it installs (plugin.cpp:842-845), the
+ * mesh, field and LBVH are replaced together once the new ones are built;
+ * on error the handle keeps its old geometry.  Batches that step the
+ * handle are planned again. */
+int hcs_replace_soft_mesh(hcs_ctx* ctx, int32_t handle, const double* V, int32_t nv, const int32_t* T, int32_t nt,
+                          const double* pressure);
+int hcs_replace_rigid_mesh(hcs_ctx* ctx, int32_t handle, const double* V, int32_t nv, const int32_t* F,
+                           int32_t ntri);
 /* info = [kind (1 soft, 2 rigid mesh, 3 rigid half-space), nv, elements,
  * bvh height] */
 int hcs_mesh_info(hcs_ctx* ctx, int32_t handle, int64_t* info);
@@ -235,6 +246,9 @@ int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
  * candidate kernel (1, or up to 8: one per XCD)]; 5 int32. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
 int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
+/* Per-replay step times: reps graph replays back to back on hcs_stream with
+ * a HIP event between consecutive ones; ms[r] = replay r in milliseconds. */
+int hcs_batch_time_steps(hcs_batch* batch, int32_t reps, double* ms);
 /* Eager per-kernel timing with HIP events on hcs_stream, averaged over
  * reps: ms[0] candidate kernel (traversal, cheap tests, compaction),
  * ms[1] clip/force kernel, ms[2] scan + per-pair reduction. */
@@ -279,6 +293,29 @@ int hcs_gen_soft(int32_t L, int32_t K, int32_t shape, double a, double b, double
  * (2 m^2 triangles, fixed diagonal), 2 icosphere of radius a, level m. */
 int hcs_gen_rigid_size(int32_t kind, int32_t m, int32_t* nv, int32_t* ntri);
 int hcs_gen_rigid(int32_t kind, int32_t m, double a, double b, double c, double* V, int32_t* F);
+/* Meshes of the MuJoCo geom types the plugin meshes for its cs:: geoms
+ * (parseMujocoCustomFields and onGeomChanged, plugin.cpp:634-742,
+ * 828-915), in place of the Drake generators called there.  type is the
+ * mjtGeom value; size is geom_size (sphere radius; ellipsoid semi-axes;
+ * cylinder radius and half length; box half sizes); resolution_hint is the
+ * cs:: value.  soft = 1: a tet mesh (elements nelem x 4) with pressure 0 on
+ * the surface and E at the deepest vertex, linear in every tet (sphere and
+ * ellipsoid: a single interior vertex; cylinder: cells around the medial
+ * axis; box: a grid of spacing <= hint when hint > 0, else cells around the
+ * medial axis); soft = 0: the closed surface (elements nelem x 3).  Plane,
+ * capsule and height field are not meshed (HCS_ERR_ARG), as in the
+ * reference; a soft cylinder needs hint > 0. */
+#define HCS_GEOM_PLANE 0
+#define HCS_GEOM_SPHERE 2
+#define HCS_GEOM_CAPSULE 3
+#define HCS_GEOM_ELLIPSOID 4
+#define HCS_GEOM_CYLINDER 5
+#define HCS_GEOM_BOX 6
+#define HCS_GEOM_MESH 7
+int hcs_gen_geom_size(int32_t type, int32_t soft, const double* size, double resolution_hint, int32_t* nv,
+                      int32_t* nelem);
+int hcs_gen_geom(int32_t type, int32_t soft, const double* size, double resolution_hint, double E, double* V,
+                 int32_t* elements, double* pressure);
 /* Soft body from a closed surface: one tet per triangle fanned to the
  * enclosed-volume centroid, pressure 0 on the surface and E at the centroid
  * (plugin.cpp:743-793 with CalcCentroidOfEnclosedVolume, plugin.cpp:161-187).

@@ -66,6 +66,14 @@ class CurvedSensor(C.Structure):
                 ("sample_idx", C.POINTER(C.c_int32)), ("weights", C.POINTER(C.c_double))]
 
 
+class MjData(C.Structure):
+    _fields_ = [("geom_xpos", C.POINTER(C.c_double)), ("geom_xmat", C.POINTER(C.c_double)),
+                ("geom_bodyid", C.POINTER(C.c_int32)), ("user", C.c_void_p),
+                ("object_velocity", C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.POINTER(C.c_double))),
+                ("apply_ft", C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double), C.c_int32))]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
@@ -81,6 +89,8 @@ SIGNATURES = {
     "hcs_add_soft_mesh": (C.c_int, [_P, _D, C.c_int32, _I32, C.c_int32, _D, C.POINTER(Props), C.c_int32, _I32]),
     "hcs_add_rigid_mesh": (C.c_int, [_P, _D, C.c_int32, _I32, C.c_int32, C.POINTER(Props), C.c_int32, _I32]),
     "hcs_add_rigid_plane": (C.c_int, [_P, C.POINTER(Props), C.c_int32, _I32]),
+    "hcs_replace_soft_mesh": (C.c_int, [_P, C.c_int32, _D, C.c_int32, _I32, C.c_int32, _D]),
+    "hcs_replace_rigid_mesh": (C.c_int, [_P, C.c_int32, _D, C.c_int32, _I32, C.c_int32]),
     "hcs_mesh_info": (C.c_int, [_P, C.c_int32, _I64]),
     "hcs_mesh_tet_records": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_contact_step": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), C.POINTER(PairOut),
@@ -99,6 +109,7 @@ SIGNATURES = {
     "hcs_batch_config": (C.c_int, [_P, _I32]),
     "hcs_batch_set_config": (C.c_int, [_P, _I32]),
     "hcs_batch_kernel_times": (C.c_int, [_P, C.c_int32, _D]),
+    "hcs_batch_time_steps": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_batch_counts": (C.c_int, [_P, _I64]),
     "hcs_batch_item_stats": (C.c_int, [_P, _I64, C.c_int64, _I64]),
     "hcs_contact_jacobian": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), _D]),
@@ -110,6 +121,14 @@ SIGNATURES = {
     "hcs_gen_rigid_size": (C.c_int, [C.c_int32, C.c_int32, _I32, _I32]),
     "hcs_gen_rigid": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, _D, _I32]),
     "hcs_gen_soft_from_surface": (C.c_int, [_D, C.c_int32, _I32, C.c_int32, C.c_double, _D, _I32, _D]),
+    "hcs_gen_geom_size": (C.c_int, [C.c_int32, C.c_int32, _D, C.c_double, _I32, _I32]),
+    "hcs_gen_geom": (C.c_int, [C.c_int32, C.c_int32, _D, C.c_double, C.c_double, _D, _I32, _D]),
+    # include/hcs_mujoco.h
+    "hcs_mj_create": (C.c_int, [_P, C.c_int32, C.POINTER(StepOpts), C.POINTER(_P)]),
+    "hcs_mj_destroy": (C.c_int, [_P]),
+    "hcs_mj_bind_geom": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "hcs_mj_collision": (C.c_int, [_P, C.POINTER(MjData), C.c_int32, C.c_int32, _I32, _I32]),
+    "hcs_mj_passive": (C.c_int, [_P, C.POINTER(MjData), C.POINTER(PairOut), _I32, C.c_int32, _I32, C.POINTER(Faces)]),
 }
 
 _lib = None

@@ -17,8 +17,9 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libhcs.so")
 
-HIP_SOURCES = ["hcs_engine.hip", "hcs_batch.hip", "hcs_taxels.hip", "hcs_taxel_sensor.hip", "hcs_adjoint.hip"]
-CXX_SOURCES = ["hcs_gen.cpp"]
+HIP_SOURCES = ["hcs_engine.hip", "hcs_batch.hip", "hcs_taxels.hip", "hcs_taxel_sensor.hip", "hcs_adjoint.hip",
+               "hcs_prim.hip"]
+CXX_SOURCES = ["hcs_gen.cpp", "hcs_gen_geom.cpp", "hcs_mujoco.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -57,6 +58,17 @@ def build_engine(jobs=8):
     return LIB
 
 
+def build_glue_demo():
+    """The compiled driver of the MuJoCo host glue used by the GPU tests
+    (tests/glue/glue_demo.cpp), linked against the in-tree libhcs.so."""
+    src = os.path.join(ROOT, "tests", "glue", "glue_demo.cpp")
+    out = os.path.join(ROOT, "tests", "glue", "hcs_glue_demo")
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o", out + ".tmp", "-L" + PKG,
+          "-lhcs", "-Wl,-rpath,$ORIGIN/../../mujoco_contact_surfaces_amd"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "-B", "liboracle.so"])
     return os.path.join(ROOT, "oracle", "liboracle.so")
@@ -64,6 +76,7 @@ def build_oracle():
 
 def main():
     print(build_engine())
+    print(build_glue_demo())
     print(build_oracle())
 
 

@@ -214,6 +214,16 @@ static int batch_settle(hcs_batch* b) {
   return set_err(c, HCS_ERR_CAPACITY, "survivor buffer did not settle");
 }
 
+int hcs::batches_refresh(Ctx* c, int handle) {
+  for (hcs_batch* b : c->batches) {
+    if (std::find(b->geoms.begin(), b->geoms.end(), handle) == b->geoms.end()) continue;
+    int rc = batch_plan(b);
+    if (!rc) rc = batch_capture(b);
+    if (rc) return rc;
+  }
+  return HCS_OK;
+}
+
 extern "C" {
 
 int hcs_batch_create(hcs_ctx* h, int32_t npairs, const int32_t* geoms, const hcs_step_opts* opts,
@@ -403,6 +413,31 @@ int hcs_batch_item_stats(hcs_batch* b, int64_t* out, int64_t capacity, int64_t* 
     o[8] = (int64_t)q[4], o[9] = (int64_t)q[5];
   }
   return HCS_OK;
+}
+
+int hcs_batch_time_steps(hcs_batch* b, int32_t reps, double* ms) {
+  if (!b || !ms || reps <= 0) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  int rc = batch_settle(b);
+  if (rc) return rc;
+  std::vector<hipEvent_t> ev(reps + 1, nullptr);
+  for (auto& e : ev)
+    if ((rc = hip_check(c, hipEventCreate(&e), "event"))) break;
+  /* back-to-back replays, an event between consecutive ones */
+  if (!rc) rc = hip_check(c, hipEventRecord(ev[0], c->stream), "record");
+  for (int r = 0; r < reps && !rc; ++r) {
+    rc = hcs_batch_step(b);
+    if (!rc) rc = hip_check(c, hipEventRecord(ev[r + 1], c->stream), "record");
+  }
+  if (!rc) rc = hip_check(c, hipEventSynchronize(ev[reps]), "sync");
+  for (int r = 0; r < reps && !rc; ++r) {
+    float t = 0;
+    rc = hip_check(c, hipEventElapsedTime(&t, ev[r], ev[r + 1]), "elapsed");
+    ms[r] = t;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
 }
 
 int hcs_batch_kernel_times(hcs_batch* b, int32_t reps, double* ms) {

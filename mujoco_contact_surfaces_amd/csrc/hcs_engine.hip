@@ -14,7 +14,6 @@
  * parseMujocoCustomFields (:656,680,704,730,790).
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -26,6 +25,7 @@
 #include <vector>
 
 #include "hcs_internal.h"
+#include "hcs_prim.h"
 #include "hcs_query.h"
 
 #pragma clang fp contract(off)
@@ -541,7 +541,7 @@ template <class T, int KS>
 __global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int dir_base, Contrib* out,
                                               int64_t out_stride) {
   const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
-                  A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan};
+                  A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan, A.poly_env};
   const OutPtrs* op = (A.face_rec || A.poly_rec) ? &o : nullptr;
   const int dir = dir_base < 0 ? -1 : dir_base + (int)blockIdx.y;
   Contrib* dst = out + (dir < 0 ? 0 : (int64_t)(dir - dir_base) * out_stride);
@@ -558,11 +558,6 @@ __global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int di
  * k_env_combine adds a pair's slices in order, so results do not depend on
  * timing, and a heavy pair (K_e grows with its item count) no longer sets
  * the time of the whole reduction. */
-struct RedPart {
-  double F[3], tA[3], tB[3];
-  int64_t np, nf, nc;
-};
-
 __global__ __launch_bounds__(64) void k_env_reduce(const EnvDesc* __restrict__ envs, const int4* __restrict__ rtab,
                                                    int nred, const unsigned* __restrict__ off,
                                                    const int* __restrict__ item_ncand,
@@ -575,7 +570,10 @@ __global__ __launch_bounds__(64) void k_env_reduce(const EnvDesc* __restrict__ e
   /* on survivor-list overflow the offsets run past the contribution slots:
    * sum nothing, the status reports the overflow */
   const bool over = (int64_t)*surv_cnt > surv_cap;
-  const unsigned lo = off[first], hi = over ? lo : off[first + cnt];
+  /* (slots never reach past the buffer; the clamp keeps every read inside
+   * it whatever the offsets hold) */
+  const unsigned capu = (unsigned)(surv_cap < 0xffffffffLL ? surv_cap : 0xffffffffLL);
+  const unsigned lo = min(off[first], capu), hi = over ? lo : max(lo, min(off[first + cnt], capu));
   const uint64_t len = hi - lo;
   const unsigned s0 = lo + (unsigned)(len * j / K), s1 = lo + (unsigned)(len * (j + 1) / K);
   const int i0 = first + (int)((int64_t)cnt * j / K), i1 = first + (int)((int64_t)cnt * (j + 1) / K);
@@ -648,9 +646,7 @@ int pipe_reserve(Ctx* c, Pipe& p, int nitems, int npairs, int64_t cap, int ndir)
       (rc = ensure(c, p.contrib, sizeof(Contrib) * cap * std::max(1, ndir))))
     return rc;
   p.cap = cap;
-  size_t bytes = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr, (int)ni + 1,
-                                   c->stream);
+  const size_t bytes = scan_temp_bytes((long long)ni + 1);
   if ((rc = ensure(c, p.scan_tmp, std::max<size_t>(bytes, 8)))) return rc;
   p.scan_bytes = bytes;
   return HCS_OK;
@@ -729,9 +725,9 @@ void launch_cand(hipStream_t s, const QueryArgs& a, int grid, int mode) {
 static void pipe_back(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, int ndir,
                       const double* dstar_dE, const double* invE, int dir0, hipEvent_t e_clip, hipEvent_t e_reduce) {
   hipStream_t s = c->stream;
-  size_t bytes = p.scan_bytes;
-  hipcub::DeviceScan::ExclusiveSum(p.scan_tmp.ptr, bytes, (const unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr,
-                                   a.nitems + 1, s);
+  /* sizes were checked when the buffers were reserved */
+  (void)exclusive_scan_u32(s, (const unsigned*)p.nsurv.ptr, (unsigned*)p.off.ptr, (long long)a.nitems + 1,
+                           p.scan_tmp.ptr, p.scan_bytes);
   if (e_clip) (void)hipEventRecord(e_clip, s);
   ClipArgs C{(const unsigned*)p.off.ptr, dstar_dE, invE, p.cap};
   const int g = std::max(1, clip_grid);
@@ -773,6 +769,19 @@ void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int cli
   pipe_back(c, p, a, npairs, clip_grid, 0, nullptr, nullptr, 0, e_clip, e_reduce);
 }
 
+/* Reduction slices: a pair of n items gets ceil(n / 8) waves (at most 64;
+ * a split heavy pair has many items).  rt: per wave (pair, slice, slices,
+ * first partial of the pair); pt: per pair (first partial, slices). */
+void reduction_slices(const Plan& plan, std::vector<int4>& rt, std::vector<int2>& pt) {
+  rt.clear();
+  pt.assign(std::max<size_t>(1, plan.envs.size()), make_int2(0, 0));
+  for (size_t e = 0; e < plan.envs.size(); ++e) {
+    const int K = std::max(1, std::min(64, (plan.envs[e].item_count + 7) / 8));
+    pt[e] = make_int2((int)rt.size(), K);
+    for (int j = 0; j < K; ++j) rt.push_back(make_int4((int)e, j, K, (int)rt.size() - j));
+  }
+}
+
 int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Plan& plan) {
   int rc;
   const std::vector<Item>& host_items = plan.items;
@@ -793,16 +802,10 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
   p.nxenv = (int)plan.xenv.size();
   p.kinds = 0;
   for (const EnvDesc& E : plan.envs) p.kinds |= 1 << E.kind;
-  /* reduction slices: a pair of n items gets ceil(n / 8) waves (at most
-   * 64); a split heavy pair has many items */
   {
     std::vector<int4> rt;
-    std::vector<int2> pt(std::max<size_t>(1, plan.envs.size()));
-    for (size_t e = 0; e < plan.envs.size(); ++e) {
-      const int K = std::max(1, std::min(64, (plan.envs[e].item_count + 7) / 8));
-      pt[e] = make_int2((int)rt.size(), K);
-      for (int j = 0; j < K; ++j) rt.push_back(make_int4((int)e, j, K, (int)rt.size() - j));
-    }
+    std::vector<int2> pt;
+    reduction_slices(plan, rt, pt);
     p.nred = (int)rt.size();
     if ((rc = ensure(c, p.rtab, sizeof(int4) * std::max<size_t>(1, rt.size()))) ||
         (rc = ensure(c, p.ptab, sizeof(int2) * pt.size())) ||
@@ -930,13 +933,32 @@ static int build_bvh(Ctx* c, Mesh& m) {
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, c->stream, m.box, n, lo[0], lo[1], lo[2],
                        sc[0], sc[1], sc[2], codes, idx);
     fail(hipGetLastError(), "k_morton");
-    fail(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, codes, codes2, idx, idx2, n, 0, 30, c->stream),
-         "SortPairs");
+    tmp_bytes = sort_temp_bytes(n, sizeof(unsigned));
     fail(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 8)), "hipMalloc");
   }
   if (rc == HCS_OK) {
-    fail(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, codes, codes2, idx, idx2, n, 0, 30, c->stream),
-         "SortPairs");
+    if (radix_sort_pairs_u32(c->stream, codes, codes2, idx, idx2, n, 0, 30, tmp, tmp_bytes) != 0)
+      fail(hipErrorInvalidValue, "radix sort size");
+  }
+  if (rc == HCS_OK) {
+    /* the leaf order must be a permutation in code order before the tree
+     * is built on it */
+    std::vector<unsigned> hc(n);
+    std::vector<int> hi(n);
+    fail(hipMemcpyAsync(hc.data(), codes2, n * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream), "memcpy");
+    fail(hipMemcpyAsync(hi.data(), idx2, n * sizeof(int), hipMemcpyDeviceToHost, c->stream), "memcpy");
+    fail(hipStreamSynchronize(c->stream), "sync");
+    if (rc == HCS_OK) {
+      std::vector<char> seen(n, 0);
+      bool ok = true;
+      for (int i = 0; i < n && ok; ++i) {
+        ok = hi[i] >= 0 && hi[i] < n && !seen[hi[i]] && (i == 0 || hc[i - 1] <= hc[i]);
+        if (ok) seen[hi[i]] = 1;
+      }
+      if (!ok) rc = set_err(c, HCS_ERR_HIP, "LBVH build: the Morton sort returned an invalid order");
+    }
+  }
+  if (rc == HCS_OK) {
     fail(hipMemsetAsync(flags, 0, n * sizeof(int), c->stream), "memset");
     hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, c->stream, codes2, idx2, n, m.nodes, np,
                        lp);
@@ -1134,16 +1156,21 @@ int hcs_synchronize(hcs_ctx* h) {
 
 void* hcs_stream(hcs_ctx* h) { return h ? (void*)h->c.stream : nullptr; }
 
-int hcs_add_soft_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* T, int32_t nt, const double* pr,
-                      const hcs_props* props, int32_t geom_id, int32_t* handle) {
-  if (!h || !V || !T || !pr || !props || !handle || nv < 4 || nt < 1) return HCS_ERR_ARG;
-  Ctx* c = &h->c;
+}  // extern "C"
+
+namespace hcs {
+
+/* The device data of a soft mesh (tet records, float face planes, boxes,
+ * LBVH); m is left empty on failure. */
+static int make_soft(Ctx* c, const double* V, int32_t nv, const int32_t* T, int32_t nt, const double* pr,
+                     const hcs_props* props, int32_t geom_id, Mesh& m) {
+  if (!V || !T || !pr || !props || nv < 4 || nt < 1) return HCS_ERR_ARG;
   if (!(props->hydroelastic_modulus > 0) || std::isinf(props->hydroelastic_modulus))
     return set_err(c, HCS_ERR_ARG, "soft mesh needs a finite hydroelastic_modulus > 0");
   for (int64_t i = 0; i < 4 * (int64_t)nt; ++i)
     if (T[i] < 0 || T[i] >= nv) return set_err(c, HCS_ERR_ARG, "tet vertex index out of range");
   HCHK(c, hipSetDevice(c->device));
-  Mesh m;
+  m = Mesh();
   m.kind = 1;
   m.geom_id = geom_id;
   m.props = *props;
@@ -1164,22 +1191,20 @@ int hcs_add_soft_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* T,
   if (rc != HCS_OK) {
     (void)hipStreamSynchronize(c->stream);
     free_mesh(m);
-    return rc;
   }
-  *handle = (int32_t)c->meshes.size();
-  c->meshes.push_back(m);
-  return HCS_OK;
+  return rc;
 }
 
-int hcs_add_rigid_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* F, int32_t ntri,
-                       const hcs_props* props, int32_t geom_id, int32_t* handle) {
-  if (!h || !V || !F || !props || !handle || nv < 3 || ntri < 1) return HCS_ERR_ARG;
-  Ctx* c = &h->c;
+/* The device data of a rigid triangle mesh (triangles in a Morton query
+ * order, unit normals); m is left empty on failure. */
+static int make_rigid(Ctx* c, const double* V, int32_t nv, const int32_t* F, int32_t ntri, const hcs_props* props,
+                      int32_t geom_id, Mesh& m) {
+  if (!V || !F || !props || nv < 3 || ntri < 1) return HCS_ERR_ARG;
   if (!is_rigid(*props)) return set_err(c, HCS_ERR_ARG, "rigid mesh needs hydroelastic_modulus 0 or +inf");
   for (int64_t i = 0; i < 3 * (int64_t)ntri; ++i)
     if (F[i] < 0 || F[i] >= nv) return set_err(c, HCS_ERR_ARG, "triangle vertex index out of range");
   HCHK(c, hipSetDevice(c->device));
-  Mesh m;
+  m = Mesh();
   m.kind = 2;
   m.geom_id = geom_id;
   m.props = *props;
@@ -1225,13 +1250,71 @@ int hcs_add_rigid_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* F
     rc = hip_check(c, hipGetLastError(), "k_tri_normals");
   }
   if (rc == HCS_OK) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
-  if (rc != HCS_OK) {
+  if (rc != HCS_OK) free_mesh(m);
+  return rc;
+}
+
+/* Installs m as the geometry of `handle` (onGeomChanged): the old device
+ * data is freed once nothing in flight uses it, and every batch that steps
+ * the handle is planned and captured again. */
+static int install_mesh(Ctx* c, int32_t handle, Mesh& m) {
+  int rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  if (rc) {
     free_mesh(m);
     return rc;
   }
-  *handle = (int32_t)c->meshes.size();
-  c->meshes.push_back(m);
+  free_mesh(c->meshes[handle]);
+  c->meshes[handle] = m;
+  c->last_valid = 0;
+  return batches_refresh(c, handle);
+}
+
+}  // namespace hcs
+
+extern "C" {
+
+int hcs_add_soft_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* T, int32_t nt, const double* pr,
+                      const hcs_props* props, int32_t geom_id, int32_t* handle) {
+  if (!h || !handle) return HCS_ERR_ARG;
+  Mesh m;
+  const int rc = make_soft(&h->c, V, nv, T, nt, pr, props, geom_id, m);
+  if (rc) return rc;
+  *handle = (int32_t)h->c.meshes.size();
+  h->c.meshes.push_back(m);
   return HCS_OK;
+}
+
+int hcs_add_rigid_mesh(hcs_ctx* h, const double* V, int32_t nv, const int32_t* F, int32_t ntri,
+                       const hcs_props* props, int32_t geom_id, int32_t* handle) {
+  if (!h || !handle) return HCS_ERR_ARG;
+  Mesh m;
+  const int rc = make_rigid(&h->c, V, nv, F, ntri, props, geom_id, m);
+  if (rc) return rc;
+  *handle = (int32_t)h->c.meshes.size();
+  h->c.meshes.push_back(m);
+  return HCS_OK;
+}
+
+int hcs_replace_soft_mesh(hcs_ctx* h, int32_t handle, const double* V, int32_t nv, const int32_t* T, int32_t nt,
+                          const double* pressure) {
+  if (!h || handle < 0 || handle >= (int)h->c.meshes.size()) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (c->meshes[handle].kind != 1) return set_err(c, HCS_ERR_ARG, "handle is not a soft mesh");
+  Mesh m;
+  const hcs_props props = c->meshes[handle].props;
+  int rc = make_soft(c, V, nv, T, nt, pressure, &props, c->meshes[handle].geom_id, m);
+  return rc ? rc : install_mesh(c, handle, m);
+}
+
+int hcs_replace_rigid_mesh(hcs_ctx* h, int32_t handle, const double* V, int32_t nv, const int32_t* F,
+                           int32_t ntri) {
+  if (!h || handle < 0 || handle >= (int)h->c.meshes.size()) return HCS_ERR_ARG;
+  Ctx* c = &h->c;
+  if (c->meshes[handle].kind != 2) return set_err(c, HCS_ERR_ARG, "handle is not a rigid triangle mesh");
+  Mesh m;
+  const hcs_props props = c->meshes[handle].props;
+  int rc = make_rigid(c, V, nv, F, ntri, &props, c->meshes[handle].geom_id, m);
+  return rc ? rc : install_mesh(c, handle, m);
 }
 
 int hcs_add_rigid_plane(hcs_ctx* h, const hcs_props* props, int32_t geom_id, int32_t* handle) {
@@ -1271,7 +1354,7 @@ int hcs_mesh_tet_records(hcs_ctx* h, int32_t handle, double* out) {
 namespace hcs {
 
 /* Uploads plan and state of npairs pairs into the context scratch. */
-int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan) {
+int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan, DBuf* dst) {
   HCHK(c, hipSetDevice(c->device));
   std::vector<int32_t> geoms(2 * (size_t)npairs);
   std::vector<double> state((size_t)HCS_STATE * npairs);
@@ -1286,15 +1369,16 @@ int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* 
   int rc = build_plan(c, npairs, geoms.data(), opts, 1, c->tet_mode_max_tris, plan);
   if (rc) return rc;
   const int nitems = (int)plan.items.size();
-  if ((rc = ensure(c, c->d_envs, sizeof(EnvDesc) * npairs)) || (rc = ensure(c, c->d_state, state.size() * 8)) ||
-      (rc = ensure(c, c->d_items, sizeof(Item) * std::max(1, nitems))))
+  DBuf& envs = dst ? dst[0] : c->d_envs;
+  DBuf& st = dst ? dst[1] : c->d_state;
+  DBuf& items = dst ? dst[2] : c->d_items;
+  if ((rc = ensure(c, envs, sizeof(EnvDesc) * npairs)) || (rc = ensure(c, st, state.size() * 8)) ||
+      (rc = ensure(c, items, sizeof(Item) * std::max(1, nitems))))
     return rc;
-  HCHK(c, hipMemcpyAsync(c->d_envs.ptr, plan.envs.data(), sizeof(EnvDesc) * npairs, hipMemcpyHostToDevice,
-                         c->stream));
-  HCHK(c, hipMemcpyAsync(c->d_state.ptr, state.data(), state.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HCHK(c, hipMemcpyAsync(envs.ptr, plan.envs.data(), sizeof(EnvDesc) * npairs, hipMemcpyHostToDevice, c->stream));
+  HCHK(c, hipMemcpyAsync(st.ptr, state.data(), state.size() * 8, hipMemcpyHostToDevice, c->stream));
   if (nitems)
-    HCHK(c, hipMemcpyAsync(c->d_items.ptr, plan.items.data(), sizeof(Item) * nitems, hipMemcpyHostToDevice,
-                           c->stream));
+    HCHK(c, hipMemcpyAsync(items.ptr, plan.items.data(), sizeof(Item) * nitems, hipMemcpyHostToDevice, c->stream));
   c->last_geoms = geoms;
   return HCS_OK;
 }

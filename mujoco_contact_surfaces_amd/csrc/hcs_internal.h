@@ -128,7 +128,16 @@ void pipe_free(Pipe& p);
 void launch_cand(hipStream_t s, const QueryArgs& a, int grid, int mode);
 void launch_clip_reduce(Ctx* c, Pipe& p, const QueryArgs& a, int npairs, int clip_grid, hipEvent_t e_clip,
                         hipEvent_t e_reduce);
-int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan);
+/* Builds the plan of npairs pairs and uploads env descriptors, state and
+ * items into the context's scratch buffers, or into dst[0..2] (envs, state,
+ * items) when given. */
+int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* opts, Plan& plan,
+                DBuf* dst = nullptr);
+/* The per-pair reduction waves of a plan (see pipe_plan). */
+void reduction_slices(const Plan& plan, std::vector<int4>& rt, std::vector<int2>& pt);
+/* Plans and captures again every batch of the context that steps mesh
+ * `handle` (its geometry was replaced). */
+int batches_refresh(Ctx* c, int handle);
 int clip_grid_for(const Ctx* c, int64_t cap);
 void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double* xf, int n);
 void launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, int n);

@@ -152,6 +152,7 @@ struct QueryArgs {
   unsigned long long* poly_cnt;
   int64_t poly_cap;
   double* poly_tan; /* tangent copy of poly_rec (tangent pass only) */
+  const unsigned char* poly_env; /* pairs whose polygons are kept (nullptr: every pair) */
   int64_t* cand_key; /* keys-only mode when non-null */
   unsigned long long* cand_cnt;
   int64_t cand_cap;
@@ -178,6 +179,18 @@ struct QueryArgs {
   int32_t kinds;         /* bit k set: some pair is of EnvKind k; a launch
                             whose pairs share one kind runs a kernel built
                             for it (0: not known, the general kernel) */
+};
+
+/* One survivor's wrench contribution (clip kernel output, 80 B). */
+struct Contrib {
+  double F[3], tA[3], tB[3];
+  int np, nf;
+};
+
+/* One reduction wave's sums over its slice of a pair's contributions. */
+struct RedPart {
+  double F[3], tA[3], tB[3];
+  int64_t np, nf, nc;
 };
 
 /* Inputs of the clip kernel beyond QueryArgs. */

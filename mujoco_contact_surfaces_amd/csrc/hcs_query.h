@@ -104,6 +104,7 @@ struct OutPtrs {
   unsigned long long* poly_cnt;
   int64_t poly_cap;
   double* poly_tan;
+  const unsigned char* poly_env; /* pairs whose polygons are kept (nullptr: every pair) */
 };
 
 /* Polygon (A frame) -> faces -> point collisions -> wrench; kTriangle fan
@@ -125,7 +126,7 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
   rot(ic.pw.X_WA, nA, nW);
   rot(ic.pw.X_WA, gA, gAW);
   const int n = p.n;
-  if (o && o->poly_rec) {
+  if (o && o->poly_rec && (!o->poly_env || o->poly_env[env])) {
     unsigned long long slot = atomicAdd(o->poly_cnt, 1ull);
     if ((int64_t)slot < o->poly_cap) {
       double* P = o->poly_rec + slot * HCS_PREC;
@@ -1055,11 +1056,6 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
 /* ===================================================================
  * k_clip: one survivor per thread -> contribution at item_off + seq
  * =================================================================== */
-
-struct Contrib {
-  double F[3], tA[3], tB[3];
-  int np, nf;
-};
 
 template <class T, int KS = -1>
 __device__ inline void clip_survivor(const QueryArgs& A, const ClipArgs& C, int64_t i, int dir, Contrib* out,

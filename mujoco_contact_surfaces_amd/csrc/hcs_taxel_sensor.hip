@@ -24,7 +24,6 @@
  * butterfly, so results are deterministic.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -33,6 +32,7 @@
 
 #include "hcs_device.h"
 #include "hcs_internal.h"
+#include "hcs_prim.h"
 #include "hcs_query.h"
 
 #pragma clang fp contract(off)
@@ -355,15 +355,13 @@ static int sensor_fans(Ctx* c, int32_t geom, unsigned& ntri) {
     hipLaunchKernelGGL(k_ts_keys, dim3((np + 255) / 256), dim3(256), 0, s, (const int64_t*)c->d_poly_key.ptr, np,
                        (const unsigned char*)B.mask.ptr, k64, kp, idx, (unsigned*)B.cnt.ptr);
     /* stable LSD passes: (element of A, element of B), then pair */
-    size_t b1 = 0, b2 = 0;
-    hipcub::DeviceRadixSort::SortPairs(nullptr, b1, k64, k64 + n1, idx, idx + n1, (int)np, 0, 64, s);
-    hipcub::DeviceRadixSort::SortPairs(nullptr, b2, kp + n1, kp, idx + n1, idx, (int)np, 0, 32, s);
+    const size_t b1 = sort_temp_bytes(np, 8), b2 = sort_temp_bytes(np, 4);
     if ((rc = ensure(c, B.tmp, std::max<size_t>(8, std::max(b1, b2))))) return rc;
-    b1 = B.tmp.bytes;
-    hipcub::DeviceRadixSort::SortPairs(B.tmp.ptr, b1, k64, k64 + n1, idx, idx + n1, (int)np, 0, 64, s);
+    if (radix_sort_pairs_u64(s, k64, k64 + n1, idx, idx + n1, np, 0, 64, B.tmp.ptr, B.tmp.bytes) != 0)
+      return set_err(c, HCS_ERR_LIMIT, "sensor: too many polygons to sort");
     hipLaunchKernelGGL(k_ts_gather_u32, dim3((np + 255) / 256), dim3(256), 0, s, kp, idx + n1, np, kp + n1);
-    b2 = B.tmp.bytes;
-    hipcub::DeviceRadixSort::SortPairs(B.tmp.ptr, b2, kp + n1, kp, idx + n1, idx, (int)np, 0, 32, s);
+    if (radix_sort_pairs_u32(s, kp + n1, kp, idx + n1, idx, np, 0, 32, B.tmp.ptr, B.tmp.bytes) != 0)
+      return set_err(c, HCS_ERR_LIMIT, "sensor: too many polygons to sort");
     rc = hip_check(c, hipGetLastError(), "select polygons");
     if (!rc) rc = hip_check(c, hipMemcpyAsync(&nsel, B.cnt.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, s), "dl");
     if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
@@ -374,11 +372,10 @@ static int sensor_fans(Ctx* c, int32_t geom, unsigned& ntri) {
   if (nsel > 0) {
     hipLaunchKernelGGL(k_ts_nfan, dim3((nsel + 1 + 255) / 256), dim3(256), 0, s, (const double*)c->d_poly_rec.ptr,
                        idx, (int)nsel, (unsigned*)B.nfan.ptr);
-    size_t sb = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, sb, (unsigned*)B.nfan.ptr, (unsigned*)B.fan_off.ptr, (int)nsel + 1, s);
-    if ((rc = ensure(c, B.tmp, std::max<size_t>(sb, 8)))) return rc;
-    sb = B.tmp.bytes;
-    hipcub::DeviceScan::ExclusiveSum(B.tmp.ptr, sb, (unsigned*)B.nfan.ptr, (unsigned*)B.fan_off.ptr, (int)nsel + 1, s);
+    if ((rc = ensure(c, B.tmp, std::max<size_t>(scan_temp_bytes((long long)nsel + 1), 8)))) return rc;
+    if (exclusive_scan_u32(s, (const unsigned*)B.nfan.ptr, (unsigned*)B.fan_off.ptr, (long long)nsel + 1, B.tmp.ptr,
+                           B.tmp.bytes) != 0)
+      return set_err(c, HCS_ERR_LIMIT, "sensor: too many polygons to scan");
     rc = hip_check(c, hipMemcpyAsync(&ntri, (unsigned*)B.fan_off.ptr + nsel, sizeof(unsigned), hipMemcpyDeviceToHost,
                                      s), "dl");
     if (!rc) rc = hip_check(c, hipStreamSynchronize(s), "sync");
@@ -413,11 +410,10 @@ int hcs_taxel_sensor_update(hcs_ctx* h, const hcs_taxel_sensor* S, double* value
   if (ntri > 0 && S->sample_method == 0) {
     hipLaunchKernelGGL(k_ts_count, dim3((ntri + 1 + 255) / 256), dim3(256), 0, s, (const double*)B.tris.ptr,
                        (int64_t)ntri, S->sample_resolution, (unsigned*)B.scnt.ptr);
-    size_t sb = 0;
-    hipcub::DeviceScan::ExclusiveSum(nullptr, sb, (unsigned*)B.scnt.ptr, (unsigned*)B.soff.ptr, (int)ntri + 1, s);
-    if ((rc = ensure(c, B.tmp, std::max<size_t>(sb, 8)))) return rc;
-    sb = B.tmp.bytes;
-    hipcub::DeviceScan::ExclusiveSum(B.tmp.ptr, sb, (unsigned*)B.scnt.ptr, (unsigned*)B.soff.ptr, (int)ntri + 1, s);
+    if ((rc = ensure(c, B.tmp, std::max<size_t>(scan_temp_bytes((long long)ntri + 1), 8)))) return rc;
+    if (exclusive_scan_u32(s, (const unsigned*)B.scnt.ptr, (unsigned*)B.soff.ptr, (long long)ntri + 1, B.tmp.ptr,
+                           B.tmp.bytes) != 0)
+      return set_err(c, HCS_ERR_LIMIT, "sensor: too many triangles to scan");
     unsigned m32 = 0;
     rc = hip_check(c, hipMemcpyAsync(&m32, (unsigned*)B.soff.ptr + ntri, sizeof(unsigned), hipMemcpyDeviceToHost, s),
                    "dl");

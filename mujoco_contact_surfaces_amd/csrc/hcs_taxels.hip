@@ -19,7 +19,6 @@
  * the canonical polygon order.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -95,14 +94,13 @@ __device__ inline Dual ld<Dual>(const double* v, const double* d, size_t i) {
   return Dual(v[i], d[i]);
 }
 
+/* The value of taxel (x, y): the block's 256 threads sample it; the block's
+ * candidate polygons are list[base .. base + n). */
 template <class T>
-__global__ __launch_bounds__(256) void k_tx_sample(const double* __restrict__ poly, const double* __restrict__ ptan,
-                                                   const int64_t* __restrict__ key, TaxelGridDev g,
-                                                   const double* __restrict__ Xd, const unsigned* __restrict__ off,
-                                                   const unsigned* __restrict__ cnt, const int* __restrict__ list,
-                                                   double* __restrict__ out) {
-  const int cell = blockIdx.x;
-  const int x = cell % g.cx, y = cell / g.cx;
+__device__ inline double sample_cell(const double* __restrict__ poly, const double* __restrict__ ptan,
+                                     const int64_t* __restrict__ key, const TaxelGridDev& g,
+                                     const double* __restrict__ Xd, unsigned base, unsigned n,
+                                     const int* __restrict__ list, int x, int y) {
   const int S = g.S;
   const double rSR = g.res / S;
   const double rmean = 1. / (S * S);
@@ -117,7 +115,6 @@ __global__ __launch_bounds__(256) void k_tx_sample(const double* __restrict__ po
   if (g.window == 1) rsigma_squared = 0.5 / (g.sigma * g.sigma);
   if (g.window == 2) rtukey = 1. / g.sigma * S * S;
   const double PI = 3.14159265358979323846;
-  const unsigned base = off[cell], n = cnt[cell];
   T acc = T(0.0);
   for (int s = threadIdx.x; s < S * S; s += blockDim.x) {
     const int i = s / S, j = s % S;
@@ -210,7 +207,19 @@ __global__ __launch_bounds__(256) void k_tx_sample(const double* __restrict__ po
     if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[x + g.cx * y] = red[0];
+  return red[0];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_tx_sample(const double* __restrict__ poly, const double* __restrict__ ptan,
+                                                   const int64_t* __restrict__ key, TaxelGridDev g,
+                                                   const double* __restrict__ Xd, const unsigned* __restrict__ off,
+                                                   const unsigned* __restrict__ cnt, const int* __restrict__ list,
+                                                   double* __restrict__ out) {
+  const int cell = blockIdx.x;
+  const int x = cell % g.cx, y = cell / g.cx;
+  const double v = sample_cell<T>(poly, ptan, key, g, Xd, off[cell], cnt[cell], list, x, y);
+  if (threadIdx.x == 0) out[x + g.cx * y] = v;
 }
 
 /* Bins polygons of the masked pairs into the taxel grid and samples it.

@@ -134,6 +134,19 @@ class Engine:
         self._kind[h.value] = "rigid"
         return h.value
 
+    def replace_soft_mesh(self, handle, V, T, p):
+        """onGeomChanged for a soft geom: new tets and pressures, same
+        properties and id (batches stepping the handle are re-planned)."""
+        V, T, p = A.as_f64(V, (-1, 3)), A.as_i32(T, (-1, 4)), A.as_f64(p)
+        self._chk(self.lib.hcs_replace_soft_mesh(self.h, int(handle), A.dptr(V), V.shape[0], A.i32ptr(T), T.shape[0],
+                                                 A.dptr(p)))
+
+    def replace_rigid_mesh(self, handle, V, F):
+        """onGeomChanged for a rigid mesh geom."""
+        V, F = A.as_f64(V, (-1, 3)), A.as_i32(F, (-1, 3))
+        self._chk(self.lib.hcs_replace_rigid_mesh(self.h, int(handle), A.dptr(V), V.shape[0], A.i32ptr(F),
+                                                  F.shape[0]))
+
     def add_rigid_plane(self, props, geom_id):
         """A MuJoCo plane geom: the rigid half-space z <= 0 of its frame."""
         h = C.c_int32()
@@ -320,6 +333,13 @@ class Batch:
     def set_config(self, blocks_per_cu, queries_per_lane, tet_query_max_tris, split_target=2048, queues=1):
         cfg = np.array([blocks_per_cu, queries_per_lane, tet_query_max_tris, split_target, queues], dtype=np.int32)
         self.e._chk(self.e.lib.hcs_batch_set_config(self.b, A.i32ptr(cfg)))
+
+    def step_times(self, reps=50):
+        """Milliseconds of each of reps back-to-back graph replays (HIP
+        events between consecutive replays)."""
+        ms = np.zeros(int(reps))
+        self.e._chk(self.e.lib.hcs_batch_time_steps(self.b, int(reps), A.dptr(ms)))
+        return ms
 
     def kernel_times(self, reps=5):
         ms = np.zeros(3)

@@ -300,3 +300,62 @@ def soft_plane_env(seed, env, nt_range=(2e3, 3e4), geom_order=0):
     return dict(soft=dict(V=V, T=T, p=p, props=dict(hydroelastic_modulus=E, **SOFT_PROPS), geom_id=gid_s),
                 plane=dict(props=dict(RIGID_PROPS), geom_id=gid_p),
                 X_WA=pose(R, c), X_WB=pose(RP, cP), V_WA=twist(rng), V_WB=twist(rng, 0.1, 0.005))
+
+
+def upload_batched(engine, n_env, seed=0, log=None, every=512):
+    """Builds the batched workload (batched_env for env 0..n_env-1) and
+    uploads every mesh.  Returns (scenes, handles (n, 2) int32, state (n, 36):
+    X_WA, X_WB, V_WA, V_WB per env)."""
+    scs, handles = [], []
+    for e in range(n_env):
+        sc = batched_env(seed, e)
+        s, r = sc["soft"], sc["rigid"]
+        ha = engine.add_soft_mesh(s["V"], s["T"], s["p"], s["props"], s["geom_id"])
+        hb = engine.add_rigid_mesh(r["V"], r["F"], r["props"], r["geom_id"])
+        scs.append(sc)
+        handles.append((ha, hb))
+        if log and (e + 1) % every == 0:
+            log("uploaded %d/%d envs" % (e + 1, n_env))
+    state = np.array([np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]]) for sc in scs])
+    return scs, np.array(handles, dtype=np.int32).reshape(-1, 2), state
+
+
+def upload_soft_soft(engine, n_pairs, seed=0, nt_range=(2e4, 2e5), log=None, every=32):
+    """Builds soft_soft_env for pairs 0..n_pairs-1 and uploads both bodies.
+    Returns (scenes, handles, state) as upload_batched."""
+    scs, handles = [], []
+    for e in range(n_pairs):
+        sc = soft_soft_env(seed, e, nt_range=nt_range)
+        a, b = sc["soft"], sc["soft_b"]
+        ha = engine.add_soft_mesh(a["V"], a["T"], a["p"], a["props"], a["geom_id"])
+        hb = engine.add_soft_mesh(b["V"], b["T"], b["p"], b["props"], b["geom_id"])
+        scs.append(sc)
+        handles.append((ha, hb))
+        if log and (e + 1) % every == 0:
+            log("uploaded %d/%d pairs" % (e + 1, n_pairs))
+    state = np.array([np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]]) for sc in scs])
+    return scs, np.array(handles, dtype=np.int32).reshape(-1, 2), state
+
+
+GEOM = dict(sphere=2, ellipsoid=4, cylinder=5, box=6)
+
+
+def geom_mesh(kind, size, resolution_hint, E=None):
+    """Mesh of a MuJoCo geom as the plugin builds it for a cs:: geom
+    (hcs_gen_geom): kind 'sphere' | 'ellipsoid' | 'cylinder' | 'box', size =
+    geom_size; with E a soft tet mesh (V, T, p), else the rigid surface
+    (V, F)."""
+    lib = A.lib()
+    t = GEOM[kind]
+    s = np.zeros(3)
+    s[:len(size)] = size
+    soft = E is not None
+    nv, ne = A.C.c_int32(), A.C.c_int32()
+    _check(lib.hcs_gen_geom_size(t, int(soft), A.dptr(s), float(resolution_hint), A.C.byref(nv), A.C.byref(ne)),
+           "hcs_gen_geom_size")
+    V = np.zeros((nv.value, 3))
+    El = np.zeros((ne.value, 4 if soft else 3), dtype=np.int32)
+    p = np.zeros(nv.value)
+    _check(lib.hcs_gen_geom(t, int(soft), A.dptr(s), float(resolution_hint), float(E if soft else 0.0), A.dptr(V),
+                            A.i32ptr(El), A.dptr(p)), "hcs_gen_geom")
+    return (V, El, p) if soft else (V, El)

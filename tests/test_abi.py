@@ -1,5 +1,6 @@
-"""The C-ABI library loads and exports exactly what include/hcs.h declares
-(no compute calls: CPU only)."""
+"""The C-ABI library loads and exports exactly what include/*.h declare
+(hcs.h: the engine; hcs_mujoco.h: the MuJoCo host glue), and the ctypes
+binding covers all of it (no compute calls: CPU only)."""
 import os
 import re
 import subprocess
@@ -10,9 +11,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_functions():
-    txt = open(os.path.join(ROOT, "include", "hcs.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return set(re.findall(r"\b(hcs_[a-z0-9_]+)\s*\(", txt))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            # declarations (a return type before the name), not the callback members of a struct
+            names |= set(re.findall(r"^(?:int|void\*|const char\*)\s+(hcs_[a-z0-9_]+)\s*\(", txt, flags=re.M))
+    return names
 
 
 def test_library_exports_every_declared_entry_point():
