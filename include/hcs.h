@@ -204,6 +204,18 @@ int hcs_mesh_tet_records(hcs_ctx* ctx, int32_t handle, double* out);
 int hcs_contact_step(hcs_ctx* ctx, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts,
                      hcs_pair_out* out, hcs_faces* faces);
 
+/* hcs_contact_step for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid
+ * hipMalloc); no counterpart in the reference.  Wrenches and counts only
+ * (no faces; keep_polygons must be 0).  Every device buffer of the step is
+ * a slice of one static 64 MiB array per device, so the call allocates and
+ * frees nothing; pairs, work items and survivors must fit it together (at
+ * most about 690k survivors of the exact rejections) or the call returns
+ * HCS_ERR_CAPACITY.  The array is shared by every call on one device:
+ * calls that overlap on one device race. */
+int hcs_contact_step_noalloc(hcs_ctx* ctx, int32_t npairs, const hcs_pair_in* in, const hcs_step_opts* opts,
+                             hcs_pair_out* out);
+
 /* Canonical candidate keys of one pair for parity checks:
  * key = elemA * n_elements(B) + elemB, sorted ascending. */
 int hcs_candidates(hcs_ctx* ctx, const hcs_pair_in* pair, int64_t* keys, int64_t capacity, int64_t* count);

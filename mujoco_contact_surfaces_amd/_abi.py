@@ -95,6 +95,8 @@ SIGNATURES = {
     "hcs_mesh_tet_records": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_contact_step": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), C.POINTER(PairOut),
                                    C.POINTER(Faces)]),
+    "hcs_contact_step_noalloc": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts),
+                                           C.POINTER(PairOut)]),
     "hcs_candidates": (C.c_int, [_P, C.POINTER(PairIn), _I64, C.c_int64, _I64]),
     "hcs_taxel_dims": (C.c_int, [C.POINTER(TaxelGrid), _I32, _I32]),
     "hcs_taxels": (C.c_int, [_P, C.c_int32, C.POINTER(TaxelGrid), _D]),

@@ -18,7 +18,7 @@ BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libhcs.so")
 
 HIP_SOURCES = ["hcs_engine.hip", "hcs_batch.hip", "hcs_taxels.hip", "hcs_taxel_sensor.hip", "hcs_adjoint.hip",
-               "hcs_prim.hip"]
+               "hcs_noalloc.hip", "hcs_prim.hip"]
 CXX_SOURCES = ["hcs_gen.cpp", "hcs_gen_geom.cpp", "hcs_mujoco.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -54,6 +54,9 @@ int hip_check(Ctx* c, hipError_t e, const char* what) {
 
 int ensure(Ctx* c, DBuf& b, size_t bytes) {
   if (bytes <= b.bytes && b.ptr) return HCS_OK;
+  if (b.fixed)
+    return set_err(c, HCS_ERR_CAPACITY, "request of " + std::to_string(bytes) + " bytes exceeds a fixed " +
+                                            std::to_string(b.bytes) + "-byte slice of the scratch array");
   if (b.ptr) {
     HCHK(c, hipStreamSynchronize(c->stream));
     HCHK(c, hipFree(b.ptr));
@@ -1122,6 +1125,12 @@ int hcs_create(hcs_ctx** out, int device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
   if (c->num_cu <= 0) c->num_cu = 256;
+  if (scratch_setup(c) != HCS_OK) {
+    g_create_err = c->err;
+    (void)hipStreamDestroy(c->stream);
+    delete h;
+    return HCS_ERR_HIP;
+  }
   *out = h;
   return HCS_OK;
 }

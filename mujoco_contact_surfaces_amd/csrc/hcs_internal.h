@@ -32,10 +32,14 @@ struct Mesh {
   double* nrm = nullptr;
 };
 
-/* Grow-only device buffer. */
+/* Grow-only device buffer.  A fixed buffer is a view of memory the library
+ * does not own (a slice of the static scratch array of the no-allocation
+ * entry point): ensure() never reallocates or frees it, and reports a
+ * request beyond its size as HCS_ERR_CAPACITY. */
 struct DBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  bool fixed = false;
 };
 
 /* The per-step pipeline (hcs_engine.hip): k_prep, k_cand, scan of
@@ -96,6 +100,9 @@ struct Ctx {
   std::vector<hcs_pair_in> last_pairs;
   std::vector<int32_t> last_geoms; /* per pair: geom_a, geom_b */
   int tet_mode_max_tris = 64;
+  /* device address and size of the static scratch array on this device */
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
   /* live batches of this context: hcs_destroy releases them first */
   std::set<hcs_batch*> batches;
 };
@@ -135,6 +142,9 @@ int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* 
                 DBuf* dst = nullptr);
 /* The per-pair reduction waves of a plan (see pipe_plan). */
 void reduction_slices(const Plan& plan, std::vector<int4>& rt, std::vector<int2>& pt);
+/* The static scratch array of the no-allocation entry point on the
+ * context's device (set up by hcs_create). */
+int scratch_setup(Ctx* c);
 /* Plans and captures again every batch of the context that steps mesh
  * `handle` (its geometry was replaced). */
 int batches_refresh(Ctx* c, int handle);
