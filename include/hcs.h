@@ -252,8 +252,9 @@ int hcs_batch_get(hcs_batch* batch, hcs_pair_out* out);
  * current state and splits heavy work items over subtrees; results never
  * depend on the plan beyond the order of floating-point sums. */
 int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
-/* Launch configuration: cfg = [persistent blocks per CU, queries per lane,
- * largest rigid mesh handled in tet-query mode, candidate count above which
+/* Launch configuration: cfg = [persistent blocks per CU, queries per lane
+ * (must be 1: larger values are rejected), largest rigid mesh handled in
+ * tet-query mode, candidate count above which
  * a work item is split over LBVH subtrees (0: never), work queues of the
  * candidate kernel (1, or up to 8: one per XCD)]; 5 int32. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);

@@ -341,14 +341,15 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
   const EnvDesc& E = envs[it.env];
   const double inf = __builtin_inf();
   double lo[3] = {inf, inf, inf}, hi[3] = {-inf, -inf, -inf};
-  if (lane < it.count) {
+  /* every query of the item (64 per lane-batch, more with queries_per_lane) */
+  for (int l = lane; l < it.count; l += 64) {
     if (E.kind == KIND_TET || E.kind == KIND_PLANE) {
-      const double* b = E.boxLA + 6 * (size_t)(it.first + lane);
+      const double* b = E.boxLA + 6 * (size_t)(it.first + l);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) lo[c] = b[c], hi[c] = b[3 + c];
+      for (int c = 0; c < 3; ++c) lo[c] = fmin(lo[c], b[c]), hi[c] = fmax(hi[c], b[3 + c]);
     } else {
       const int nvq = E.kind == KIND_SOFT ? 4 : 3;
-      const int q = it.first + lane;
+      const int q = it.first + l;
       const int32_t* f = E.FB + (size_t)nvq * (E.qorder ? E.qorder[q] : q);
       for (int k = 0; k < nvq; ++k) {
         const double* v = E.VB + 3 * (size_t)f[k];

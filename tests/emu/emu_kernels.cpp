@@ -20,6 +20,10 @@
 thread_local Dim3 threadIdx;
 thread_local int g_lane;
 WaveEmu* g_wave;
+/* emu_set_options: run the general candidate kernel (the one mixed-kind
+ * launches use) instead of the kind's own, and items of 64 q queries on top
+ * of a positive split depth */
+int g_generic = 0, g_qpl = 0;
 
 using namespace hcs;
 
@@ -138,7 +142,9 @@ extern "C" {
  *         2 soft-soft (B = tets VB/FB with pressures pB), 3 rigid half-space
  *   params: E_A, d_A, E_B, d_B, mu_A, mu_B, dt, apply, representation, a_is_M
  *           (E_B = +inf, d_B = 1 for a rigid B)
- *   split: LBVH frontier depth of the items (0: whole tree; kinds 0 and 2)
+ *   split: LBVH frontier depth of the items (0: whole tree; kinds 0 and 2);
+ *          a negative split -q instead gives items of 64 q queries (kinds 0
+ *          and 2: one wave runs them 64 at a time)
  *   ndir, seeds: tangent directions 0..ndir-1 (0-2 rotation of B about the
  *           world axes, 3-5 translation of B, 6 modulus of A), seeds =
  *           [d(d*)/dE_A, 1/E_A]
@@ -181,7 +187,7 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
   const int nq = chunks_of_A ? nta : nb;
   std::vector<int> roots{0};
   if (!chunks_of_A && nta > 1)
-    for (int d = 0; d < split; ++d) {
+    for (int d = 0; d < std::max(split, 0); ++d) {
       std::vector<int> nx;
       for (int r : roots) {
         if (r < 0) nx.push_back(r);
@@ -190,8 +196,9 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
       roots.swap(nx);
     }
   std::vector<Item> items;
-  for (int q = 0; q < nq; q += 64)
-    for (int r : roots) items.push_back(Item{0, q, std::min(64, nq - q), chunks_of_A ? 0 : r});
+  const int per = chunks_of_A ? 64 : (split < 0 ? 64 * -split : 64 * std::max(1, g_qpl));
+  for (int q = 0; q < nq; q += per)
+    for (int r : roots) items.push_back(Item{0, q, std::min(per, nq - q), chunks_of_A ? 0 : r});
   E.item_first = 0, E.item_count = (int)items.size();
   const int nitems = (int)items.size();
   double X[12];
@@ -257,8 +264,8 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
         unsigned long long sn = 0, se = 0;
         /* the kernels the engine launches: built for the pair's kind for
          * triangle and soft queries, the general one otherwise */
-        if (E.kind == KIND_TRI) run_cand<CAND_STEP, KIND_TRI>(*L, Q, it, items[it], sn, se);
-        else if (E.kind == KIND_SOFT) run_cand<CAND_STEP, KIND_SOFT>(*L, Q, it, items[it], sn, se);
+        if (E.kind == KIND_TRI && !g_generic) run_cand<CAND_STEP, KIND_TRI>(*L, Q, it, items[it], sn, se);
+        else if (E.kind == KIND_SOFT && !g_generic) run_cand<CAND_STEP, KIND_SOFT>(*L, Q, it, items[it], sn, se);
         else run_cand<CAND_STEP>(*L, Q, it, items[it], sn, se);
         fill_sentinels(Q, sn, se);
       });
@@ -274,8 +281,8 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
   for (int dir = -1; dir < ndir; ++dir) {
     for (auto& q : con) q = Contrib{};
     for (unsigned long long i = 0; i < scnt; ++i) {
-      if (dir < 0 && E.kind == KIND_TRI) clip_survivor<double, KIND_TRI>(Q, C, (int64_t)i, dir, con.data(), nullptr);
-      else if (dir < 0 && E.kind == KIND_SOFT)
+      if (dir < 0 && E.kind == KIND_TRI && !g_generic) clip_survivor<double, KIND_TRI>(Q, C, (int64_t)i, dir, con.data(), nullptr);
+      else if (dir < 0 && E.kind == KIND_SOFT && !g_generic)
         clip_survivor<double, KIND_SOFT>(Q, C, (int64_t)i, dir, con.data(), nullptr);
       else if (dir < 0) clip_survivor<double>(Q, C, (int64_t)i, dir, con.data(), nullptr);
       else clip_survivor<Dual>(Q, C, (int64_t)i, dir, con.data(), nullptr);
@@ -300,6 +307,8 @@ extern "C" {
 
 /* The separating-axis prefilter of the candidate kernel on one box
  * (lo[3], hi[3]) and one triangle (3 x 3 vertices, unit normal n). */
+void emu_set_options(int generic, int qpl) { g_generic = generic, g_qpl = qpl; }
+
 int emu_box_may_meet_triangle(const double* box, const double* tri, const double* n) {
   return box_may_meet_triangle(box, tri, n) ? 1 : 0;
 }

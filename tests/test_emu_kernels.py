@@ -41,6 +41,8 @@ def emu(tmp_path_factory):
                              C.c_int, C.c_int, _D, _D, _I64, _I64]
     lib.emu_box_may_meet_triangle.restype = C.c_int
     lib.emu_box_may_meet_triangle.argtypes = [_D, _D, _D]
+    lib.emu_set_options.restype = None
+    lib.emu_set_options.argtypes = [C.c_int, C.c_int]
     return lib
 
 
@@ -128,8 +130,11 @@ def _scene(kind):
     return S.soft_plane_env(3, 0, nt_range=(2e3, 6e3), geom_order=1)
 
 
-@pytest.mark.parametrize("kind,split", [(KTRI, 0), (KTRI, 3), (KTET, 0), (KSOFT, 0), (KSOFT, 2), (KPLANE, 0)])
+@pytest.mark.parametrize("kind,split", [(KTRI, 0), (KTRI, 3), (KTRI, -3), (KTET, 0), (KSOFT, 0), (KSOFT, 2),
+                                        (KSOFT, -2), (KPLANE, 0)])
 def test_kernel_bodies_match_the_oracle(emu, kind, split):
+    """split > 0: items over LBVH subtrees; split = -q: items of 64 q
+    queries (queries_per_lane q)."""
     sc = _scene(kind)
     W, counts, stats = run_pair(emu, sc, kind, split=split, representation=split % 2)
     ref = oracle(sc, kind, representation=split % 2)
@@ -138,8 +143,25 @@ def test_kernel_bodies_match_the_oracle(emu, kind, split):
     assert_wrench_close(W[0], ref["wrench"])
     assert stats[2] == 0, "an item rejected by its static boxes has candidates"
     assert stats[4] == 0, "traversal error"
-    if split:
+    if split > 0:
         assert stats[1] > 0  # subtree items away from the contact are culled
+
+
+@pytest.mark.parametrize("kind,split,qpl", [(KTRI, 0, 0), (KTRI, 2, 2), (KSOFT, 0, 0), (KSOFT, 2, 3)])
+def test_general_kernel_matches_the_oracle(emu, kind, split, qpl):
+    """The general (not kind-specialised) candidate and clip bodies, which
+    launches over pairs of mixed kinds run; with qpl > 0, items of 64 qpl
+    queries on top of the subtree split."""
+    sc = _scene(kind)
+    emu.emu_set_options(1, qpl)
+    try:
+        W, counts, stats = run_pair(emu, sc, kind, split=split)
+    finally:
+        emu.emu_set_options(0, 0)
+    ref = oracle(sc, kind)
+    np.testing.assert_array_equal(counts, ref["counts"])
+    assert_wrench_close(W[0], ref["wrench"])
+    assert stats[2] == 0 and stats[4] == 0
 
 
 def _rotated(X, d, h):

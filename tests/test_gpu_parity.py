@@ -248,6 +248,11 @@ def test_batched_matches_plain_and_oracle(engine):
         for i in range(len(scs)):
             assert_wrench_close(W4[i], W[i], rtol=1e-12)
     assert b.config()["split_target"] == 8
+    # items of more than 64 queries are switched off: rejected, plan kept
+    from mujoco_contact_surfaces_amd.engine import HcsError
+    with pytest.raises(HcsError, match="queries_per_lane"):
+        b.set_config(2, 2, 0, 8)
+    assert b.config()["queries_per_lane"] == 1
     b.close()
 
 
