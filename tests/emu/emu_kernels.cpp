@@ -305,10 +305,28 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
 
 extern "C" {
 
-/* The separating-axis prefilter of the candidate kernel on one box
- * (lo[3], hi[3]) and one triangle (3 x 3 vertices, unit normal n). */
 void emu_set_options(int generic, int qpl) { g_generic = generic, g_qpl = qpl; }
 
+/* A wave whose even and odd lanes reach two different ballots: the
+ * emulation must abort on it (a collective under divergent control flow). */
+int emu_divergent_ballot_probe() {
+  WaveEmu wave;
+  g_wave = &wave;
+  std::vector<std::thread> th;
+  unsigned long long r[64];
+  for (int lane = 0; lane < 64; ++lane)
+    th.emplace_back([&, lane] {
+      g_lane = lane;
+      threadIdx.x = lane;
+      if (lane & 1) r[lane] = __ballot(1);
+      else r[lane] = __ballot(0);
+    });
+  for (auto& t : th) t.join();
+  return (int)(r[0] & 1);
+}
+
+/* The separating-axis prefilter of the candidate kernel on one box
+ * (lo[3], hi[3]) and one triangle (3 x 3 vertices, unit normal n). */
 int emu_box_may_meet_triangle(const double* box, const double* tri, const double* n) {
   return box_may_meet_triangle(box, tri, n) ? 1 : 0;
 }

@@ -12,6 +12,7 @@ oracle within 1e-6 (the differentiation path, switched off on the GPU).
 import ctypes as C
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -43,6 +44,7 @@ def emu(tmp_path_factory):
     lib.emu_box_may_meet_triangle.argtypes = [_D, _D, _D]
     lib.emu_set_options.restype = None
     lib.emu_set_options.argtypes = [C.c_int, C.c_int]
+    lib.emu_divergent_ballot_probe.restype = C.c_int
     return lib
 
 
@@ -162,6 +164,16 @@ def test_general_kernel_matches_the_oracle(emu, kind, split, qpl):
     np.testing.assert_array_equal(counts, ref["counts"])
     assert_wrench_close(W[0], ref["wrench"])
     assert stats[2] == 0 and stats[4] == 0
+
+
+def test_divergent_collective_aborts_the_emulation(emu):
+    """Every ballot, shuffle and barrier checks that all 64 lanes came from
+    one source site; lanes split over two ballots abort (run in a child
+    process, since the emulation calls abort)."""
+    code = ("import ctypes, sys; lib = ctypes.CDLL(sys.argv[1]); lib.emu_divergent_ballot_probe()")
+    r = subprocess.run([sys.executable, "-c", code, emu._name], capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "divergent collective" in r.stderr
 
 
 def _rotated(X, d, h):
