@@ -19,7 +19,8 @@ timed steps is reported beside it).  The same JSON line carries:
     algorithmic bytes of its units, and of the whole step;
   * the per-call ("plain") path on the same environments, and the other
     configurations: soft sphere on a box (us per step), tactile pad with
-    16 x 16 taxels (us per step), 256 soft-soft pairs (tet-tet clips/s);
+    16 x 16 taxels (us per step), 1024 batched pads each with its 16 x 16
+    sensor (ms per step), 256 soft-soft pairs (tet-tet clips/s);
   * the CPU baseline: the oracle's per-step work on this host (serial and on
     the host's CPU share), whose results are compared env by env with the
     GPU's (counts bit-exact, wrenches within 1e-10).
@@ -192,10 +193,10 @@ def median_ms(fn, reps):
     return float(np.median(ts))
 
 
-def extra_configs(engine, n_soft_soft=256):
+def extra_configs(engine, n_soft_soft=256, n_tactile=1024):
     """The other configurations: per-call latency of a soft sphere on a box
-    and of the tactile pad with its 16 x 16 taxels, and batched soft-soft
-    throughput."""
+    and of the tactile pad with its 16 x 16 taxels, batched tactile pads with
+    their sensors, and batched soft-soft throughput."""
     from mujoco_contact_surfaces_amd import scenes as S
     out = {}
     sc = S.sphere_box(0)
@@ -227,6 +228,37 @@ def extra_configs(engine, n_soft_soft=256):
                               tris=int(r["F"].shape[0]), clips=int(cnt[0, 0]), polygons=int(cnt[0, 1]),
                               taxels=256, rays_per_taxel=400,
                               path="plain per-call hcs_contact_step + hcs_taxels, host planning included")
+    # the differentiation workload's forward half: pads from its generator
+    # (<= 20k tets, <= 2k triangles), each with a 16 x 16 sensor, contact
+    # step and taxels of every env in one graph replay
+    t0 = time.time()
+    handles4, state4, grids4 = [], [], []
+    for e in range(n_tactile):
+        sc = S.differentiation_env(0, e)
+        s, r = sc["soft"], sc["rigid"]
+        ha = engine.add_soft_mesh(s["V"], s["T"], s["p"], s["props"], s["geom_id"])
+        hb = engine.add_rigid_mesh(r["V"], r["F"], r["props"], r["geom_id"])
+        handles4.append((ha, hb))
+        state4.append(np.concatenate([sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"]]))
+        grids4.append(dict(sc["grid"], geom=hb))
+    b = engine.batch(handles4)
+    b.set_state(np.array(state4))
+    b.step()
+    W, cnt = b.get()
+    b.tune(reps=1)
+    ms_contact = float(np.median(b.step_times(MIN_REPLAYS)))
+    b.set_sensors(list(range(n_tactile)), grids4)
+    b.step()
+    W2, cnt2 = b.get()
+    tx = b.taxels()
+    assert np.array_equal(cnt, cnt2)
+    ms = float(np.median(b.step_times(MIN_REPLAYS)))
+    out["tactile_batch"] = dict(envs=n_tactile, taxels=int(sum(t.size for t in tx)), rays_per_taxel=400,
+                                clips_per_step=int(cnt[:, 0].sum()), polygons_per_step=int(cnt[:, 1].sum()),
+                                ms_per_step=ms, us_per_env_step=1e3 * ms / n_tactile, ms_contact_only=ms_contact,
+                                autotune=b.config(), setup_s=time.time() - t0,
+                                path="batched graph replay: contact step + flat sensors of every env")
+    b.close()
     t0 = time.time()
     scs, handles, state = S.upload_soft_soft(engine, n_soft_soft, seed=0)
     b = engine.batch(handles)

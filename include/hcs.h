@@ -275,6 +275,21 @@ int hcs_batch_kernel_times(hcs_batch* batch, int32_t reps, double* ms);
 int hcs_batch_item_stats(hcs_batch* batch, int64_t* out, int64_t capacity, int64_t* nitems);
 /* Work-unit counts of the last step: [candidates, polygons, faces]. */
 int hcs_batch_counts(hcs_batch* batch, int64_t* counts);
+/* Flat tactile sensors inside the batch step (FlatTactileSensor::bvh_update,
+ * S/src/flat_tactile_sensor.cpp:262-402, for every environment in the same
+ * graph replay).  Sensor s belongs to pair pair[s]: as each environment is
+ * its own world, it samples only that pair's contact polygons, and its pose
+ * at every step is the pose in the batch state of the pair's geometry whose
+ * mesh handle is grids[s].geom (B when both are; X_W_sensor is not read).
+ * Sampling, windows and taxel order are those of hcs_taxels.  nsensors = 0
+ * removes the sensors. */
+int hcs_batch_set_sensors(hcs_batch* batch, int32_t nsensors, const int32_t* pair, const hcs_taxel_grid* grids);
+/* Taxels of all sensors: sum over sensors of cx * cy. */
+int hcs_batch_taxel_count(hcs_batch* batch, int64_t* ntaxels);
+/* Taxels of the last step, sensor after sensor, index x + cx * y within a
+ * sensor (grows the polygon and list buffers and re-runs the step when the
+ * step overflowed them). */
+int hcs_batch_get_taxels(hcs_batch* batch, double* out);
 
 /* ---- differentiation (no reference counterpart).  Tangent-linear
  * recomputation of one step at fixed clipping topology:

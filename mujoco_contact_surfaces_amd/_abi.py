@@ -113,6 +113,9 @@ SIGNATURES = {
     "hcs_batch_kernel_times": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_batch_time_steps": (C.c_int, [_P, C.c_int32, _D]),
     "hcs_batch_counts": (C.c_int, [_P, _I64]),
+    "hcs_batch_set_sensors": (C.c_int, [_P, C.c_int32, _I32, C.POINTER(TaxelGrid)]),
+    "hcs_batch_taxel_count": (C.c_int, [_P, _I64]),
+    "hcs_batch_get_taxels": (C.c_int, [_P, _D]),
     "hcs_batch_item_stats": (C.c_int, [_P, _I64, C.c_int64, _I64]),
     "hcs_contact_jacobian": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), _D]),
     "hcs_contact_backward": (C.c_int, [_P, C.c_int32, C.POINTER(PairIn), C.POINTER(StepOpts), _D, _D]),

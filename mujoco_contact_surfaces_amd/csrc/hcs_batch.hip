@@ -36,6 +36,7 @@ struct hcs_batch {
   bool has_state = false;
   DBuf envs, state, items;
   Pipe pipe;
+  BatchTaxels tx; /* flat tactile sensors (hcs_batch_set_sensors) */
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -54,14 +55,25 @@ static int grid_of(const hcs_batch* b) {
 }
 
 static QueryArgs batch_args(hcs_batch* b) {
-  return pipe_args(b->pipe, (const EnvDesc*)b->envs.ptr, (const double*)b->state.ptr, (const Item*)b->items.ptr,
-                   (int)b->plan.items.size(), true);
+  QueryArgs a = pipe_args(b->pipe, (const EnvDesc*)b->envs.ptr, (const double*)b->state.ptr,
+                          (const Item*)b->items.ptr, (int)b->plan.items.size(), true);
+  if (b->tx.n > 0) {
+    /* the clip kernel keeps the polygons of the pairs that carry a sensor */
+    a.poly_rec = (double*)b->tx.poly_rec.ptr;
+    a.poly_key = (int64_t*)b->tx.poly_key.ptr;
+    a.poly_cnt = (unsigned long long*)b->tx.poly_cnt.ptr;
+    a.poly_cap = b->tx.poly_cap;
+    a.poly_env = (const unsigned char*)b->tx.mask.ptr;
+  }
+  return a;
 }
 
 /* Enqueues one step on the context stream. */
 static void batch_enqueue(hcs_batch* b, const QueryArgs& a) {
+  if (b->tx.n > 0) launch_fill_u32(b->c->stream, (unsigned*)b->tx.poly_cnt.ptr, 0u, 2);
   pipe_enqueue(b->c, b->pipe, a, b->npairs, grid_of(b), clip_grid_for(b->c, b->pipe.cap), 0, nullptr, nullptr,
                false, 0);
+  batch_taxels_enqueue(b->c, b->tx, (const double*)b->state.ptr);
 }
 
 /* Uploads envs and items of b->plan and computes the static item boxes. */
@@ -196,22 +208,30 @@ static int batch_capture(hcs_batch* b) {
   return hip_check(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "GraphInstantiate");
 }
 
-/* Survivor count of the last step; grows the buffer and re-runs the step
- * when it overflowed.  Returns HCS_OK or an error. */
+/* Survivor count (and, with sensors, kept polygons and taxel list entries)
+ * of the last step; grows what overflowed and re-runs the step.  Returns
+ * HCS_OK or an error. */
 static int batch_settle(hcs_batch* b) {
   Ctx* c = b->c;
-  for (int attempt = 0; attempt < 3; ++attempt) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
     unsigned long long n = 0;
     int rc = hip_check(c, hipMemcpyAsync(&n, b->pipe.surv_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream), "dl");
     if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
     if (rc) return rc;
-    if ((int64_t)n <= b->pipe.cap) return HCS_OK;
-    rc = pipe_reserve(c, b->pipe, (int)b->plan.items.size(), b->npairs, (int64_t)n + (int64_t)n / 4 + 4096, 0);
+    int64_t npoly = 0, nlist = 0;
+    if ((int64_t)n <= b->pipe.cap && (rc = batch_taxels_demand(c, b->tx, npoly, nlist))) return rc;
+    const bool over_tx = b->tx.n > 0 && (npoly > b->tx.poly_cap || nlist > b->tx.list_cap);
+    if ((int64_t)n <= b->pipe.cap && !over_tx) return HCS_OK;
+    if ((int64_t)n > b->pipe.cap)
+      rc = pipe_reserve(c, b->pipe, (int)b->plan.items.size(), b->npairs, (int64_t)n + (int64_t)n / 4 + 4096, 0);
+    else
+      rc = batch_taxels_reserve(c, b->tx, std::max(b->tx.poly_cap, npoly + npoly / 4 + 4096),
+                                std::max(b->tx.list_cap, nlist + nlist / 4 + 4096));
     if (!rc) rc = batch_capture(b);
     if (!rc) rc = hcs_batch_step(b);
     if (rc) return rc;
   }
-  return set_err(c, HCS_ERR_CAPACITY, "survivor buffer did not settle");
+  return set_err(c, HCS_ERR_CAPACITY, "survivor or taxel buffers did not settle");
 }
 
 int hcs::batches_refresh(Ctx* c, int handle) {
@@ -270,6 +290,7 @@ int hcs_batch_destroy(hcs_batch* b) {
   for (DBuf* d : {&b->envs, &b->state, &b->items})
     if (d->ptr) (void)hipFree(d->ptr);
   pipe_free(b->pipe);
+  batch_taxels_free(b->tx);
   b->c->batches.erase(b);
   delete b;
   return HCS_OK;
@@ -315,6 +336,49 @@ int hcs_batch_counts(hcs_batch* b, int64_t* counts) {
   int rc = hcs_batch_get(b, o.data());
   counts[0] = counts[1] = counts[2] = 0;
   for (auto& r : o) counts[0] += r.n_candidates, counts[1] += r.n_polygons, counts[2] += r.n_faces;
+  return rc;
+}
+
+int hcs_batch_set_sensors(hcs_batch* b, int32_t nsensors, const int32_t* pair, const hcs_taxel_grid* grids) {
+  if (!b || nsensors < 0 || (nsensors > 0 && (!pair || !grids))) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  int rc = hip_check(c, hipSetDevice(c->device), "hipSetDevice");
+  if (rc) return rc;
+  (void)hipStreamSynchronize(c->stream);
+  batch_free_graph(b);
+  if (nsensors == 0) {
+    batch_taxels_free(b->tx);
+  } else {
+    rc = batch_taxels_setup(c, b->tx, b->npairs, b->geoms.data(), nsensors, pair, grids);
+    if (!rc)
+      rc = batch_taxels_reserve(c, b->tx, std::max<int64_t>(b->tx.poly_cap, 65536),
+                                std::max<int64_t>(b->tx.list_cap, 4 * (int64_t)b->tx.ncell + 65536));
+    if (rc) {
+      const std::string msg = c->err;
+      batch_taxels_free(b->tx);
+      (void)batch_capture(b);
+      return set_err(c, rc, msg);
+    }
+  }
+  return batch_capture(b);
+}
+
+int hcs_batch_taxel_count(hcs_batch* b, int64_t* ntaxels) {
+  if (!b || !ntaxels) return HCS_ERR_ARG;
+  *ntaxels = b->tx.ncell;
+  return HCS_OK;
+}
+
+int hcs_batch_get_taxels(hcs_batch* b, double* out) {
+  if (!b || (b->tx.ncell > 0 && !out)) return HCS_ERR_ARG;
+  Ctx* c = b->c;
+  if (b->tx.n == 0) return set_err(c, HCS_ERR_STATE, "the batch has no sensors (hcs_batch_set_sensors)");
+  int rc = batch_settle(b);
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(out, b->tx.out.ptr, sizeof(double) * b->tx.ncell, hipMemcpyDeviceToHost,
+                                     c->stream),
+                   "download taxels");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
   return rc;
 }
 
@@ -480,11 +544,13 @@ int hcs_batch_kernel_times(hcs_batch* b, int32_t reps, double* ms) {
   for (int r = 0; r < reps && !rc; ++r) {
     QueryArgs a = batch_args(b);
     Pipe& P = b->pipe;
+    if (b->tx.n > 0) launch_fill_u32(s, (unsigned*)b->tx.poly_cnt.ptr, 0u, 2);
     pipe_front(c, P, a, b->npairs);
     (void)hipEventRecord(ev[0], s);
     launch_cand(s, a, grid_of(b), CAND_STEP);
     (void)hipEventRecord(ev[1], s);
     launch_clip_reduce(c, P, a, b->npairs, clip_grid_for(c, P.cap), ev[2], ev[4]);
+    batch_taxels_enqueue(c, b->tx, (const double*)b->state.ptr);
     (void)hipEventRecord(ev[3], s);
     rc = hip_check(c, hipEventSynchronize(ev[3]), "sync");
     float t0 = 0, t1 = 0, t2 = 0, t3 = 0;

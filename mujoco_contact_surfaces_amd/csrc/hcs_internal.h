@@ -161,6 +161,33 @@ int taxel_grid_dev(const hcs_taxel_grid& G, TaxelGridDev& g);
 int run_taxels(Ctx* c, const TaxelGridDev& g, const unsigned char* d_mask, const double* poly, const double* ptan,
                const int64_t* key, int64_t np, const double* Xd, double* d_out);
 
+/* Flat tactile sensors of a batch (hcs_batch_set_sensors): sensor s samples
+ * the polygons of its pair only, at the pose of one of the pair's two
+ * geometries in the batch state.  Every buffer is sized when the sensors are
+ * set, except the polygon records and the per-taxel polygon lists, which
+ * grow when a step overflows them (checked after the step). */
+struct BatchTaxels {
+  int n = 0;     /* sensors */
+  int ncell = 0; /* taxels of all sensors */
+  DBuf grid /* TaxelGridDev per sensor, pose written per step */, src /* int per sensor: 36 pair + 12 side */,
+      cell_base /* int[n + 1] */, cell_sensor /* int per taxel */, psoff /* int[npairs + 1] */, psens /* int[n] */,
+      mask /* uchar per pair: has a sensor */, poly_rec, poly_key, poly_cnt /* ull */, cnt /* unsigned[ncell + 1] */,
+      cursor /* unsigned[ncell] */, off /* unsigned[ncell + 1] */, scan_tmp, list /* int[list_cap] */,
+      out /* double per taxel */;
+  int64_t poly_cap = 0, list_cap = 0;
+  size_t scan_bytes = 0;
+  int bin_grid = 1;
+};
+int batch_taxels_setup(Ctx* c, BatchTaxels& t, int npairs, const int32_t* geoms, int n, const int32_t* pair,
+                       const hcs_taxel_grid* grids);
+int batch_taxels_reserve(Ctx* c, BatchTaxels& t, int64_t poly_cap, int64_t list_cap);
+/* Sensor poses from the state, binning and sampling of the kept polygons
+ * (after the clip kernel of the same step). */
+void batch_taxels_enqueue(Ctx* c, BatchTaxels& t, const double* state);
+/* Polygons and list entries the last step asked for (to detect overflow). */
+int batch_taxels_demand(Ctx* c, BatchTaxels& t, int64_t& npoly, int64_t& nlist);
+void batch_taxels_free(BatchTaxels& t);
+
 }  // namespace hcs
 
 struct hcs_ctx {

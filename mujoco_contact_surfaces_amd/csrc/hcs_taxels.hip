@@ -27,6 +27,7 @@
 
 #include "hcs_device.h"
 #include "hcs_internal.h"
+#include "hcs_prim.h"
 
 #pragma clang fp contract(off)
 
@@ -272,6 +273,196 @@ int taxel_grid_dev(const hcs_taxel_grid& G, TaxelGridDev& g) {
   g.xs = G.half_size[0], g.ys = G.half_size[1], g.zs = G.half_size[2], g.res = G.resolution, g.sigma = G.sigma;
   g.S = G.sampling_resolution, g.window = G.window, g.cx = cx, g.cy = cy;
   return HCS_OK;
+}
+
+/* ===================================================================
+ * Batched sensors: every sensor of a batch in one pass per stage, inside
+ * the step's graph (no host round trip; list sizes are checked after the
+ * step, which is re-run when a buffer overflowed).
+ * =================================================================== */
+
+/* Sensor poses: the 12 doubles (rotation, position) at src[s] in the state. */
+__global__ void k_txb_pose(TaxelGridDev* __restrict__ g, const int* __restrict__ src, const double* __restrict__ state,
+                           int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 12 * n) return;
+  const int s = t / 12, k = t % 12;
+  g[s].X[k] = state[src[s] + k];
+}
+
+/* Polygons strided over the grid; each polygon goes to every sensor of its
+ * pair, into the taxels its footprint covers.  FILL = 0 counts per taxel;
+ * FILL = 1 writes the polygon into the taxel's list slice. */
+template <int FILL>
+__global__ __launch_bounds__(256) void k_txb_bin(const double* __restrict__ poly, const int64_t* __restrict__ key,
+                                                 const unsigned long long* __restrict__ poly_cnt, int64_t poly_cap,
+                                                 const int* __restrict__ psoff, const int* __restrict__ psens,
+                                                 const TaxelGridDev* __restrict__ grids,
+                                                 const int* __restrict__ cell_base, unsigned* __restrict__ cnt,
+                                                 const unsigned* __restrict__ off, unsigned* __restrict__ cursor,
+                                                 int* __restrict__ list, int64_t list_cap) {
+  const int64_t np = min((int64_t)*poly_cnt, poly_cap);
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < np; q += (int64_t)gridDim.x * blockDim.x) {
+    const int e = (int)key[3 * q];
+    const double* P = poly + q * HCS_PREC;
+    for (int k = psoff[e]; k < psoff[e + 1]; ++k) {
+      const int s = psens[k];
+      const TaxelGridDev& g = grids[s];
+      int x0, x1, y0, y1;
+      footprint(P, g, x0, x1, y0, y1);
+      for (int y = y0; y <= y1; ++y)
+        for (int x = x0; x <= x1; ++x) {
+          const int cell = cell_base[s] + x + g.cx * y;
+          if (!FILL) {
+            atomicAdd(cnt + cell, 1u);
+          } else {
+            const int64_t slot = (int64_t)off[cell] + atomicAdd(cursor + cell, 1u);
+            if (slot < list_cap) list[slot] = (int)q;
+          }
+        }
+    }
+  }
+}
+
+/* One workgroup per taxel of any sensor. */
+__global__ __launch_bounds__(256) void k_txb_sample(const double* __restrict__ poly, const int64_t* __restrict__ key,
+                                                    const TaxelGridDev* __restrict__ grids,
+                                                    const int* __restrict__ cell_base,
+                                                    const int* __restrict__ cell_sensor,
+                                                    const unsigned* __restrict__ off,
+                                                    const unsigned* __restrict__ cnt, const int* __restrict__ list,
+                                                    int64_t list_cap, double* __restrict__ out) {
+  const int cell = blockIdx.x;
+  const int s = cell_sensor[cell];
+  const unsigned n = cnt[cell], base = off[cell];
+  /* an overflowed list is re-run after the step with a larger buffer */
+  if (n == 0 || (int64_t)base + n > list_cap) {
+    if (threadIdx.x == 0) out[cell] = 0.0;
+    return;
+  }
+  const TaxelGridDev g = grids[s];
+  const int local = cell - cell_base[s];
+  const double v = sample_cell<double>(poly, nullptr, key, g, nullptr, base, n, list, local % g.cx, local / g.cx);
+  if (threadIdx.x == 0) out[cell] = v;
+}
+
+int batch_taxels_setup(Ctx* c, BatchTaxels& t, int npairs, const int32_t* geoms, int n, const int32_t* pair,
+                       const hcs_taxel_grid* grids) {
+  std::vector<TaxelGridDev> g(std::max(1, n));
+  std::vector<int> src(std::max(1, n)), base(n + 1, 0), psoff(npairs + 1, 0), psens(std::max(1, n));
+  std::vector<unsigned char> mask(std::max(1, npairs), 0);
+  for (int s = 0; s < n; ++s) {
+    const int e = pair[s];
+    if (e < 0 || e >= npairs) return set_err(c, HCS_ERR_ARG, "sensor " + std::to_string(s) + ": pair out of range");
+    if (taxel_grid_dev(grids[s], g[s]) != HCS_OK)
+      return set_err(c, HCS_ERR_ARG, "sensor " + std::to_string(s) + ": invalid taxel grid");
+    /* the pose of the pair's geometry with the sensor's mesh handle (B when
+     * both are) */
+    int side;
+    if (geoms[2 * e + 1] == grids[s].geom) side = 1;
+    else if (geoms[2 * e] == grids[s].geom) side = 0;
+    else return set_err(c, HCS_ERR_ARG, "sensor " + std::to_string(s) + ": its geom is not in its pair");
+    src[s] = HCS_STATE * e + 12 * side;
+    if ((int64_t)base[s] + (int64_t)g[s].cx * g[s].cy > (1 << 30))
+      return set_err(c, HCS_ERR_ARG, "too many taxels in the batch's sensors");
+    base[s + 1] = base[s] + g[s].cx * g[s].cy;
+    ++psoff[e + 1];
+    mask[e] = 1;
+  }
+  for (int e = 0; e < npairs; ++e) psoff[e + 1] += psoff[e];
+  {
+    std::vector<int> fill(psoff.begin(), psoff.end() - 1);
+    for (int s = 0; s < n; ++s) psens[fill[pair[s]]++] = s;
+  }
+  const int ncell = base[n];
+  std::vector<int> cs(std::max(1, ncell));
+  for (int s = 0; s < n; ++s)
+    for (int k = base[s]; k < base[s + 1]; ++k) cs[k] = s;
+  int rc;
+  if ((rc = ensure(c, t.grid, sizeof(TaxelGridDev) * g.size())) || (rc = ensure(c, t.src, sizeof(int) * src.size())) ||
+      (rc = ensure(c, t.cell_base, sizeof(int) * base.size())) ||
+      (rc = ensure(c, t.cell_sensor, sizeof(int) * cs.size())) || (rc = ensure(c, t.psoff, sizeof(int) * psoff.size())) ||
+      (rc = ensure(c, t.psens, sizeof(int) * psens.size())) || (rc = ensure(c, t.mask, mask.size())) ||
+      (rc = ensure(c, t.poly_cnt, 64)) || (rc = ensure(c, t.cnt, sizeof(unsigned) * (ncell + 1))) ||
+      (rc = ensure(c, t.cursor, sizeof(unsigned) * std::max(1, ncell))) ||
+      (rc = ensure(c, t.off, sizeof(unsigned) * (ncell + 1))) || (rc = ensure(c, t.out, sizeof(double) * std::max(1, ncell))))
+    return rc;
+  const size_t sb = scan_temp_bytes((long long)ncell + 1);
+  if ((rc = ensure(c, t.scan_tmp, std::max<size_t>(sb, 8)))) return rc;
+  t.scan_bytes = sb;
+  struct Up {
+    DBuf& d;
+    const void* h;
+    size_t bytes;
+  } ups[] = {{t.grid, g.data(), sizeof(TaxelGridDev) * g.size()}, {t.src, src.data(), sizeof(int) * src.size()},
+             {t.cell_base, base.data(), sizeof(int) * base.size()}, {t.cell_sensor, cs.data(), sizeof(int) * cs.size()},
+             {t.psoff, psoff.data(), sizeof(int) * psoff.size()}, {t.psens, psens.data(), sizeof(int) * psens.size()},
+             {t.mask, mask.data(), mask.size()}};
+  for (const Up& u : ups)
+    if ((rc = hip_check(c, hipMemcpyAsync(u.d.ptr, u.h, u.bytes, hipMemcpyHostToDevice, c->stream), "upload sensors")))
+      return rc;
+  t.n = n, t.ncell = ncell;
+  t.bin_grid = std::max(1, c->num_cu * 4);
+  return hip_check(c, hipStreamSynchronize(c->stream), "sync");
+}
+
+int batch_taxels_reserve(Ctx* c, BatchTaxels& t, int64_t poly_cap, int64_t list_cap) {
+  poly_cap = std::max<int64_t>(poly_cap, 4096);
+  list_cap = std::max<int64_t>(list_cap, 4096);
+  /* list offsets are 32-bit */
+  if (list_cap > 0xffffffffLL) return set_err(c, HCS_ERR_CAPACITY, "taxel polygon lists exceed 2^32 entries");
+  int rc;
+  if ((rc = ensure(c, t.poly_rec, sizeof(double) * HCS_PREC * poly_cap)) ||
+      (rc = ensure(c, t.poly_key, sizeof(int64_t) * 3 * poly_cap)) || (rc = ensure(c, t.list, sizeof(int) * list_cap)))
+    return rc;
+  t.poly_cap = poly_cap, t.list_cap = list_cap;
+  return HCS_OK;
+}
+
+void batch_taxels_enqueue(Ctx* c, BatchTaxels& t, const double* state) {
+  if (t.n == 0) return;
+  hipStream_t s = c->stream;
+  TaxelGridDev* g = (TaxelGridDev*)t.grid.ptr;
+  hipLaunchKernelGGL(k_txb_pose, dim3((12 * t.n + 255) / 256), dim3(256), 0, s, g, (const int*)t.src.ptr, state, t.n);
+  launch_fill_u32(s, (unsigned*)t.cnt.ptr, 0u, t.ncell + 1);
+  launch_fill_u32(s, (unsigned*)t.cursor.ptr, 0u, t.ncell);
+  const unsigned long long* pc = (const unsigned long long*)t.poly_cnt.ptr;
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_txb_bin<0>), dim3(t.bin_grid), dim3(256), 0, s, (const double*)t.poly_rec.ptr,
+                     (const int64_t*)t.poly_key.ptr, pc, t.poly_cap, (const int*)t.psoff.ptr, (const int*)t.psens.ptr,
+                     (const TaxelGridDev*)g, (const int*)t.cell_base.ptr, (unsigned*)t.cnt.ptr,
+                     (const unsigned*)nullptr, (unsigned*)nullptr, (int*)nullptr, (int64_t)0);
+  (void)exclusive_scan_u32(s, (const unsigned*)t.cnt.ptr, (unsigned*)t.off.ptr, (long long)t.ncell + 1,
+                           t.scan_tmp.ptr, t.scan_bytes);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_txb_bin<1>), dim3(t.bin_grid), dim3(256), 0, s, (const double*)t.poly_rec.ptr,
+                     (const int64_t*)t.poly_key.ptr, pc, t.poly_cap, (const int*)t.psoff.ptr, (const int*)t.psens.ptr,
+                     (const TaxelGridDev*)g, (const int*)t.cell_base.ptr, (unsigned*)t.cnt.ptr,
+                     (const unsigned*)t.off.ptr, (unsigned*)t.cursor.ptr, (int*)t.list.ptr, t.list_cap);
+  if (t.ncell > 0)
+    hipLaunchKernelGGL(k_txb_sample, dim3(t.ncell), dim3(256), 0, s, (const double*)t.poly_rec.ptr,
+                       (const int64_t*)t.poly_key.ptr, (const TaxelGridDev*)g, (const int*)t.cell_base.ptr,
+                       (const int*)t.cell_sensor.ptr, (const unsigned*)t.off.ptr, (const unsigned*)t.cnt.ptr,
+                       (const int*)t.list.ptr, t.list_cap, (double*)t.out.ptr);
+}
+
+int batch_taxels_demand(Ctx* c, BatchTaxels& t, int64_t& npoly, int64_t& nlist) {
+  npoly = nlist = 0;
+  if (t.n == 0) return HCS_OK;
+  unsigned long long np = 0;
+  unsigned nl = 0;
+  int rc = hip_check(c, hipMemcpyAsync(&np, t.poly_cnt.ptr, 8, hipMemcpyDeviceToHost, c->stream), "dl");
+  if (!rc)
+    rc = hip_check(c, hipMemcpyAsync(&nl, (const unsigned*)t.off.ptr + t.ncell, 4, hipMemcpyDeviceToHost, c->stream),
+                   "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  npoly = (int64_t)np, nlist = (int64_t)nl;
+  return rc;
+}
+
+void batch_taxels_free(BatchTaxels& t) {
+  for (DBuf* b : {&t.grid, &t.src, &t.cell_base, &t.cell_sensor, &t.psoff, &t.psens, &t.mask, &t.poly_rec, &t.poly_key,
+                  &t.poly_cnt, &t.cnt, &t.cursor, &t.off, &t.scan_tmp, &t.list, &t.out})
+    if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
+  t = BatchTaxels();
 }
 
 }  // namespace hcs

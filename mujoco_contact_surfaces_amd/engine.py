@@ -12,6 +12,7 @@ passive_callback runs the whole step on the GPU and returns one wrench per
 geometry, sensors sample the kept contact polygons); Engine is the direct
 C-ABI wrapper.  Everything runs through libhcs.so; there is no CPU path.
 """
+import atexit
 import ctypes as C
 import weakref
 
@@ -22,6 +23,22 @@ from . import _abi as A
 KTRIANGLE = 0
 KPOLYGON = 1
 WINDOWS = {"none": 0, "gauss": 1, "tukey": 2, "square": 3}
+
+
+# Engines still open when the interpreter exits are closed (batches first,
+# then the context) by an exit handler, while every module is intact, rather
+# than by garbage collection in whatever order interpreter teardown reaches
+# them.
+_live_engines = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_engines():
+    for e in list(_live_engines):
+        try:
+            e.close()
+        except Exception:
+            pass
 
 
 class HcsError(RuntimeError):
@@ -90,6 +107,7 @@ class Engine:
         self.h = h
         self._kind = {}
         self._batches = weakref.WeakSet()
+        _live_engines.add(self)
 
     def _chk(self, rc):
         if rc != 0:
@@ -355,6 +373,38 @@ class Batch:
         out = np.zeros((max(1, n.value), 10), dtype=np.int64)
         self.e._chk(self.e.lib.hcs_batch_item_stats(self.b, A.i64ptr(out), out.shape[0], C.byref(n)))
         return out[:n.value]
+
+    def set_sensors(self, pairs, grids):
+        """Flat tactile sensors inside the batch step: sensor s belongs to
+        pair pairs[s] and takes the pose of that pair's geometry grids[s]
+        ['geom'] from the batch state (X_W_sensor is not needed)."""
+        pairs = A.as_i32(pairs, (-1,))
+        gs = []
+        self._dims = []
+        for g in grids:
+            g = dict(g)
+            g.setdefault("X_W_sensor", np.r_[np.eye(3).reshape(9), 0.0, 0.0, 0.0])
+            gg = _grid(**g)
+            cx, cy = C.c_int32(), C.c_int32()
+            self.e._chk(self.e.lib.hcs_taxel_dims(C.byref(gg), C.byref(cx), C.byref(cy)))
+            self._dims.append((cx.value, cy.value))
+            gs.append(gg)
+        if len(gs) != pairs.shape[0]:
+            raise ValueError("one pair per sensor")
+        arr = (A.TaxelGrid * max(1, len(gs)))(*gs)
+        self.e._chk(self.e.lib.hcs_batch_set_sensors(self.b, len(gs), A.i32ptr(pairs), arr))
+
+    def taxels(self):
+        """Taxels of the last step, one (cy, cx) array per sensor."""
+        n = C.c_int64()
+        self.e._chk(self.e.lib.hcs_batch_taxel_count(self.b, C.byref(n)))
+        out = np.zeros(max(1, n.value))
+        self.e._chk(self.e.lib.hcs_batch_get_taxels(self.b, A.dptr(out)))
+        res, o = [], 0
+        for cx, cy in self._dims:
+            res.append(out[o:o + cx * cy].reshape(cy, cx).copy())
+            o += cx * cy
+        return res
 
     def counts(self):
         c = np.zeros(3, dtype=np.int64)

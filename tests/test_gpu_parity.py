@@ -130,7 +130,7 @@ def test_contact_and_taxels_tactile_pad(engine):
     assert_wrench_close(W[0], ref["wrench"])
     match_faces(rec, key, ref["faces"], ref["face_keys"])
     g = dict(sc["grid"])
-    for window, sigma in ((0, 0.0), (1, 0.3), (3, 0.0)):
+    for window, sigma in ((0, 0.0), (1, 0.3), (2, 0.5), (3, 0.0)):
         say("taxels window", window)
         g.update(window=window, sigma=sigma)
         tx = engine.taxels([dict(geom=hb, X_W_sensor=sc["X_WB"], **g)])[0]

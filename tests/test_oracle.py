@@ -365,6 +365,17 @@ def test_taxels_windows_weights():
     r = np.hypot(off[:, None], off[None, :]) / (math.sqrt(2) * 0.05)
     np.testing.assert_allclose(sq, 1000.0 * ((1 - r) ** 2).mean(), rtol=1e-9)
     np.testing.assert_allclose(gs, 1000.0 * np.exp(-(r ** 2) * 0.5 / 0.25).mean(), rtol=1e-9)
+    # Tukey window: the reference scales the sample distance n from the cell
+    # edge by S^2 / sigma (flat_tactile_sensor.cpp:313, 367-379), so with
+    # S = 10, sigma = 0.4 every n <= 2 gets 0.5 (1 - cos(2 pi n 250)) = 0 and
+    # 5 of 10 samples per axis survive with weight 1: 1000 * (5/10)^2.  (The
+    # textbook scaling n / (sigma S) would give per-axis weights 0, .5, 1, 1,
+    # 1, 1, 1, 1, 1, .5: 640.)  With sigma = 0.3 only n <= 1 is tapered:
+    # w(0) = 0, w(1) = 0.5 (1 - cos(2 pi / 3)) = 0.75, so (8.5 / 10)^2.
+    tk = O.taxels(res["polys"], Xs, (0.2, 0.2, 0.002), 0.1, S, window=2, sigma=0.4)
+    np.testing.assert_allclose(tk, 250.0, rtol=1e-12)
+    tk3 = O.taxels(res["polys"], Xs, (0.2, 0.2, 0.002), 0.1, S, window=2, sigma=0.3)
+    np.testing.assert_allclose(tk3, 722.5, rtol=1e-12)
 
 
 def test_taxel_output_index_is_x_plus_cx_y():
