@@ -189,7 +189,7 @@ static int batch_split(hcs_batch* b) {
  * set, heavy items are split and all items ordered by measured steps. */
 static int batch_plan(hcs_batch* b) {
   batch_free_graph(b);
-  int rc = build_plan(b->c, b->npairs, b->geoms.data(), &b->opts, b->qpl, b->tet_max, b->plan);
+  int rc = build_plan(b->c, b->npairs, b->geoms.data(), &b->opts, b->tet_max, b->plan);
   b->plan.order.clear();
   if (!rc) rc = batch_upload(b);
   if (!rc && b->has_state) rc = batch_split(b);
@@ -392,9 +392,7 @@ int hcs_batch_set_config(hcs_batch* b, const int32_t* cfg) {
   if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0 || cfg[3] < 0 ||
       cfg[4] < 1 || cfg[4] > HCS_QUEUES)
     return HCS_ERR_ARG;
-  /* Items of more than 64 queries (run 64 at a time by one wave) match the
-   * one-wave items on the CPU emulation of the kernel but gave different
-   * candidate counts on gfx950, for a cause not yet found: switched off. */
+  /* an item holds one query per lane of the candidate kernel's wave */
   if (cfg[1] != 1)
     return set_err(b->c, HCS_ERR_ARG, "queries_per_lane must be 1 (items of more than 64 queries are switched off)");
   b->blocks_per_cu = cfg[0], b->qpl = cfg[1], b->tet_max = cfg[2], b->split_target = cfg[3], b->queues = cfg[4];

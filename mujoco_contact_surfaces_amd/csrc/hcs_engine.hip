@@ -1009,13 +1009,13 @@ static void free_mesh(Mesh& m) {
   m = Mesh();
 }
 
-int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int qpl, int tet_max, Plan& plan) {
+int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int tet_max, Plan& plan) {
   plan.envs.assign(npairs, EnvDesc{});
   plan.items.clear();
   plan.xenv.clear();
   plan.ntx = 0;
   plan.pair_kind.assign(npairs, 0);
-  const int per_item = 64 * std::max(1, qpl);
+  const int per_item = 64;
   for (int e = 0; e < npairs; ++e) {
     const int ha = geoms[2 * e], hb = geoms[2 * e + 1];
     if (ha < 0 || hb < 0 || ha >= (int)c->meshes.size() || hb >= (int)c->meshes.size() ||
@@ -1376,7 +1376,7 @@ int stage_pairs(Ctx* c, int npairs, const hcs_pair_in* in, const hcs_step_opts* 
     std::memcpy(st + 24, in[e].V_WA, 6 * sizeof(double));
     std::memcpy(st + 30, in[e].V_WB, 6 * sizeof(double));
   }
-  int rc = build_plan(c, npairs, geoms.data(), opts, 1, c->tet_mode_max_tris, plan);
+  int rc = build_plan(c, npairs, geoms.data(), opts, c->tet_mode_max_tris, plan);
   if (rc) return rc;
   const int nitems = (int)plan.items.size();
   DBuf& envs = dst ? dst[0] : c->d_envs;

@@ -107,10 +107,10 @@ struct Ctx {
   std::set<hcs_batch*> batches;
 };
 
-/* Builds env descriptors and work items; tri-query chunks of 64 *
- * queries_per_lane queries.  Returns HCS_OK or an error. */
-int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int queries_per_lane,
-               int tet_mode_max_tris, Plan& plan);
+/* Builds env descriptors and work items (chunks of 64 queries, one per
+ * lane of the candidate kernel's wave).  Returns HCS_OK or an error. */
+int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o, int tet_mode_max_tris,
+               Plan& plan);
 
 int pipe_reserve(Ctx* c, Pipe& p, int nitems, int npairs, int64_t cap, int ndir);
 QueryArgs pipe_args(const Pipe& p, const EnvDesc* envs, const double* state, const Item* items, int nitems,

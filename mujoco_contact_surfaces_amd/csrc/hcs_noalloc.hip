@@ -105,7 +105,7 @@ extern "C" int hcs_contact_step_noalloc(hcs_ctx* h, int32_t npairs, const hcs_pa
   std::vector<int32_t> geoms(2 * (size_t)npairs);
   for (int e = 0; e < npairs; ++e) geoms[2 * e] = in[e].geom_a, geoms[2 * e + 1] = in[e].geom_b;
   Plan plan;
-  if ((rc = build_plan(c, npairs, geoms.data(), opts, 1, c->tet_mode_max_tris, plan))) return rc;
+  if ((rc = build_plan(c, npairs, geoms.data(), opts, c->tet_mode_max_tris, plan))) return rc;
   const size_t ni = std::max<size_t>(1, plan.items.size());
   std::vector<int4> rt;
   std::vector<int2> pt;

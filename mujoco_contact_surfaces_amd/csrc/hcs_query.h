@@ -786,19 +786,13 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
     if (hit) L.q1[q1n + lane_prefix(m)] = make_int2(b, 0);
     q1n += __popcll(m);
     producing = false;
-  }
-  /* triangle / soft queries: lanes take queries q0 + lane of the item, 64
-   * at a time (an item of 64 * queries_per_lane queries runs its batches
-   * one after the other) */
-  int q0 = 0;
-  auto start_queries = [&]() {
+  } else {
     /* b is the lane's query slot; its element comes through qorder (a
      * spatially compact order of B's elements), and candidates carry the
      * slot until the exact rejections turn it into the element */
-    const int q = item.first + q0 + lane;
-    valid = q0 + lane < item.count;
+    valid = lane < item.count;
     b = lane;
-    const int e = E.qorder ? E.qorder[valid ? q : 0] : q;
+    const int e = E.qorder ? E.qorder[item.first + lane < E.nB ? item.first + lane : 0] : item.first + lane;
     L.qb[lane] = e;
     if (valid) {
       const int nvq = kind == KIND_SOFT ? 4 : 3;
@@ -840,8 +834,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
       L.stk[lane] = item.root;
       bp = 0, sp = 1, on = 0;
     }
-  };
-  if (!tet_mode && !plane_mode) start_queries();
+  }
   for (;;) {
     /* ---- produce; in tet mode the queued candidates refer to the current
      * tile's triangles in LDS, so they are drained before the next tile */
@@ -1046,16 +1039,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
       }
       __syncthreads();
     }
-    if (!producing) {
-      /* every candidate of this batch of queries is consumed (q1 is empty),
-       * so the LDS slots can take the next batch */
-      if (tet_mode || plane_mode || q0 + 64 >= item.count) break;
-      q0 += 64;
-      __syncthreads();
-      start_queries();
-      __syncthreads();
-      producing = true;
-    }
+    if (!producing) break;
   }
   const int snc = (int)wave_sum_i(nc);
   if (lane == 0) {

@@ -95,6 +95,82 @@ __device__ inline Dual ld<Dual>(const double* v, const double* d, size_t i) {
   return Dual(v[i], d[i]);
 }
 
+/* Polygons of one chunk of a taxel's list, staged in LDS for its rays: per
+ * fan triangle (v_k, v_k+1, C) the sample-index ranges [i0, i1] x [j0, j1]
+ * (i in the low, i1 / j1 in the high 16 bits) outside of which no ray of the
+ * taxel can hit it, so a ray runs the Moeller-Trumbore test only on the fans
+ * whose range holds it. */
+#define HCS_TX_CHUNK 128
+struct TxStage {
+  uint2 rng[HCS_TX_CHUNK][8];
+  int q[HCS_TX_CHUNK], nv[HCS_TX_CHUNK];
+  int64_t kq[HCS_TX_CHUNK];
+};
+
+/* The sample-index ranges of the fans of polygon P for taxel (x, y).
+ *  - |a| < 1e-10 (a = e1 . (D x e2), the same expression the hit test
+ *    rejects on): no ray can hit; empty range.
+ *  - |a| < 1e-3 max(|e1|, |e2|)^2 (nearly edge-on to the rays): every ray.
+ *  - otherwise the rays whose sensor-plane position lies within m of the
+ *    fan's projected box, m = 1e-9 (1 + extent + height gap).  For such a
+ *    triangle the computed barycentrics err by less than ~7e-13 (L + |s|) / L
+ *    (L the longest edge, s = origin - v0), so an accepted ray lies within
+ *    ~7e-13 (L + |s|) of the projected triangle: m is three orders wider, and
+ *    the floor / ceil below add up to one more sample on each side. */
+__device__ inline void stage_fans(const double* P, const TaxelGridDev& g, int x, int y, uint2* rng) {
+  const int S = g.S;
+  const double rSR = g.res / S;
+  const int nv = min((int)P[0], 8);
+  const double Dv[3] = {-g.X[2], -g.X[5], -g.X[8]};
+  double px[9], py[9], pz[9];
+  for (int k = 0; k <= nv && k < 9; ++k) {
+    const double* v = k < nv ? P + 1 + 3 * k : P + 33;
+    const double d[3] = {v[0] - g.X[9], v[1] - g.X[10], v[2] - g.X[11]};
+    const int o = k < nv ? k : 8;
+    px[o] = (g.X[0] * d[0] + g.X[3] * d[1]) + g.X[6] * d[2];
+    py[o] = (g.X[1] * d[0] + g.X[4] * d[1]) + g.X[7] * d[2];
+    pz[o] = (g.X[2] * d[0] + g.X[5] * d[1]) + g.X[8] * d[2];
+  }
+  const double ox = -g.xs + x * g.res + 0.5 * rSR, oy = -g.ys + y * g.res + 0.5 * rSR, oz = 1.5 * g.zs;
+  for (int k = 0; k < nv; ++k) {
+    const int k1 = (k + 1 == nv) ? 0 : k + 1;
+    const double* v0 = P + 1 + 3 * k;
+    const double* v1 = P + 1 + 3 * k1;
+    const double* v2 = P + 33;
+    double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    double h[3];
+    cross3(Dv, e2, h);
+    const double a = dot3(e1, h);
+    const double L2 = fmax(dot3(e1, e1), dot3(e2, e2));
+    uint2 r;
+    if (fabs(a) < 1e-10) {
+      r = make_uint2(1u, 1u); /* i0 = 1 > i1 = 0: empty */
+    } else if (fabs(a) < 1e-3 * L2) {
+      r = make_uint2((unsigned)(S - 1) << 16, (unsigned)(S - 1) << 16);
+    } else {
+      const int id[3] = {k, k1, 8};
+      double lx = 1e300, hx = -1e300, ly = 1e300, hy = -1e300, gap = 0.0;
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        lx = fmin(lx, px[id[t]]), hx = fmax(hx, px[id[t]]);
+        ly = fmin(ly, py[id[t]]), hy = fmax(hy, py[id[t]]);
+        gap = fmax(gap, fabs(oz - pz[id[t]]));
+      }
+      const double m = 1e-9 * (1.0 + fmax(hx - lx, hy - ly) + gap);
+      /* clamp in double before the conversions */
+      const double fi0 = fmax(floor((lx - m - ox) / rSR), 0.0), fi1 = fmin(ceil((hx + m - ox) / rSR), S - 1.0);
+      const double fj0 = fmax(floor((ly - m - oy) / rSR), 0.0), fj1 = fmin(ceil((hy + m - oy) / rSR), S - 1.0);
+      if (!(fi0 <= fi1) || !(fj0 <= fj1)) {
+        r = make_uint2(1u, 1u);
+      } else {
+        r = make_uint2((unsigned)fi0 | ((unsigned)fi1 << 16), (unsigned)fj0 | ((unsigned)fj1 << 16));
+      }
+    }
+    rng[k] = r;
+  }
+}
+
 /* The value of taxel (x, y): the block's 256 threads sample it; the block's
  * candidate polygons are list[base .. base + n). */
 template <class T>
@@ -102,6 +178,7 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
                                      const int64_t* __restrict__ key, const TaxelGridDev& g,
                                      const double* __restrict__ Xd, unsigned base, unsigned n,
                                      const int* __restrict__ list, int x, int y) {
+  __shared__ TxStage L;
   const int S = g.S;
   const double rSR = g.res / S;
   const double rmean = 1. / (S * S);
@@ -117,8 +194,12 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
   if (g.window == 2) rtukey = 1. / g.sigma * S * S;
   const double PI = 3.14159265358979323846;
   T acc = T(0.0);
-  for (int s = threadIdx.x; s < S * S; s += blockDim.x) {
-    const int i = s / S, j = s % S;
+  /* rays in rounds of blockDim.x (every thread runs every round, so the
+   * staging barriers are reached by all) */
+  for (int s0 = 0; s0 < S * S; s0 += blockDim.x) {
+    const int s = s0 + (int)threadIdx.x;
+    const bool active = s < S * S;
+    const int i = active ? s / S : 0, j = active ? s % S : 0;
     T loc[3] = {T(-g.xs + x * g.res + i * rSR + 0.5 * rSR), T(-g.ys + y * g.res + j * rSR + 0.5 * rSR),
                 T(1.5 * g.zs)};
     T pt[3];
@@ -129,37 +210,55 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
     double best_t = 1e30;
     int64_t bkey = INT64_MAX;
     int bk = 0, bq = -1;
-    for (unsigned l = 0; l < n; ++l) {
-      const int q = list[base + l];
-      const double* P = poly + (size_t)q * HCS_PREC;
-      const int nv = (int)P[0];
-      const int64_t kq = pkey(key, q);
-      double Ov[3] = {val(O[0]), val(O[1]), val(O[2])}, Dv[3] = {val(D[0]), val(D[1]), val(D[2])};
-      for (int k = 0; k < nv; ++k) {
-        const int k1 = (k + 1 == nv) ? 0 : k + 1;
-        const double* v0 = P + 1 + 3 * k;
-        const double* v1 = P + 1 + 3 * k1;
-        const double* v2 = P + 33;
-        double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
-        double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
-        double h[3];
-        cross3(Dv, e2, h);
-        const double a = dot3(e1, h);
-        if (fabs(a) < 1e-10) continue;
-        const double f = 1.0 / a;
-        double sv[3] = {Ov[0] - v0[0], Ov[1] - v0[1], Ov[2] - v0[2]};
-        const double u = f * dot3(sv, h);
-        if (u < 0.0 || u > 1.0) continue;
-        double qv[3];
-        cross3(sv, e1, qv);
-        const double v = f * dot3(Dv, qv);
-        if (v < 0.0 || u + v > 1.0) continue;
-        const double t = f * dot3(e2, qv);
-        if (!(t > 0.0)) continue;
-        const bool better = t < best_t || (t == best_t && (kq < bkey || (kq == bkey && k < bk)));
-        if (better) best_t = t, bkey = kq, bk = k, bq = q;
+    const double Ov[3] = {val(O[0]), val(O[1]), val(O[2])}, Dv[3] = {val(D[0]), val(D[1]), val(D[2])};
+    for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
+      const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
+      __syncthreads();
+      for (unsigned l = threadIdx.x; l < cn; l += blockDim.x) {
+        const int q = list[base + c0 + l];
+        const double* P = poly + (size_t)q * HCS_PREC;
+        L.q[l] = q;
+        L.nv[l] = min((int)P[0], 8);
+        L.kq[l] = pkey(key, q);
+        stage_fans(P, g, x, y, L.rng[l]);
+      }
+      __syncthreads();
+      if (!active) continue;
+      for (unsigned l = 0; l < cn; ++l) {
+        const int nv = L.nv[l];
+        const int q = L.q[l];
+        const int64_t kq = L.kq[l];
+        const double* P = poly + (size_t)q * HCS_PREC;
+        for (int k = 0; k < nv; ++k) {
+          const uint2 r = L.rng[l][k];
+          if (i < (int)(r.x & 0xffffu) || i > (int)(r.x >> 16) || j < (int)(r.y & 0xffffu) || j > (int)(r.y >> 16))
+            continue;
+          const int k1 = (k + 1 == nv) ? 0 : k + 1;
+          const double* v0 = P + 1 + 3 * k;
+          const double* v1 = P + 1 + 3 * k1;
+          const double* v2 = P + 33;
+          double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+          double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+          double h[3];
+          cross3(Dv, e2, h);
+          const double a = dot3(e1, h);
+          if (fabs(a) < 1e-10) continue;
+          const double f = 1.0 / a;
+          double sv[3] = {Ov[0] - v0[0], Ov[1] - v0[1], Ov[2] - v0[2]};
+          const double u = f * dot3(sv, h);
+          if (u < 0.0 || u > 1.0) continue;
+          double qv[3];
+          cross3(sv, e1, qv);
+          const double v = f * dot3(Dv, qv);
+          if (v < 0.0 || u + v > 1.0) continue;
+          const double t = f * dot3(e2, qv);
+          if (!(t > 0.0)) continue;
+          const bool better = t < best_t || (t == best_t && (kq < bkey || (kq == bkey && k < bk)));
+          if (better) best_t = t, bkey = kq, bk = k, bq = q;
+        }
       }
     }
+    if (!active) continue;
     if (bq >= 0 && best_t < 1.5 * g.zs && best_t > 0.0) {
       /* Moeller-Trumbore on the chosen triangle, in T arithmetic */
       const size_t pb = (size_t)bq * HCS_PREC;

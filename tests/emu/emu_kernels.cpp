@@ -21,9 +21,8 @@ thread_local Dim3 threadIdx;
 thread_local int g_lane;
 WaveEmu* g_wave;
 /* emu_set_options: run the general candidate kernel (the one mixed-kind
- * launches use) instead of the kind's own, and items of 64 q queries on top
- * of a positive split depth */
-int g_generic = 0, g_qpl = 0;
+ * launches use) instead of the kind's own */
+int g_generic = 0;
 
 using namespace hcs;
 
@@ -142,9 +141,7 @@ extern "C" {
  *         2 soft-soft (B = tets VB/FB with pressures pB), 3 rigid half-space
  *   params: E_A, d_A, E_B, d_B, mu_A, mu_B, dt, apply, representation, a_is_M
  *           (E_B = +inf, d_B = 1 for a rigid B)
- *   split: LBVH frontier depth of the items (0: whole tree; kinds 0 and 2);
- *          a negative split -q instead gives items of 64 q queries (kinds 0
- *          and 2: one wave runs them 64 at a time)
+ *   split: LBVH frontier depth of the items (0: whole tree; kinds 0 and 2)
  *   ndir, seeds: tangent directions 0..ndir-1 (0-2 rotation of B about the
  *           world axes, 3-5 translation of B, 6 modulus of A), seeds =
  *           [d(d*)/dE_A, 1/E_A]
@@ -196,7 +193,7 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
       roots.swap(nx);
     }
   std::vector<Item> items;
-  const int per = chunks_of_A ? 64 : (split < 0 ? 64 * -split : 64 * std::max(1, g_qpl));
+  const int per = 64;
   for (int q = 0; q < nq; q += per)
     for (int r : roots) items.push_back(Item{0, q, std::min(per, nq - q), chunks_of_A ? 0 : r});
   E.item_first = 0, E.item_count = (int)items.size();
@@ -305,7 +302,7 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
 
 extern "C" {
 
-void emu_set_options(int generic, int qpl) { g_generic = generic, g_qpl = qpl; }
+void emu_set_options(int generic) { g_generic = generic; }
 
 /* A wave whose even and odd lanes reach two different ballots: the
  * emulation must abort on it (a collective under divergent control flow). */

@@ -43,7 +43,7 @@ def emu(tmp_path_factory):
     lib.emu_box_may_meet_triangle.restype = C.c_int
     lib.emu_box_may_meet_triangle.argtypes = [_D, _D, _D]
     lib.emu_set_options.restype = None
-    lib.emu_set_options.argtypes = [C.c_int, C.c_int]
+    lib.emu_set_options.argtypes = [C.c_int]
     lib.emu_divergent_ballot_probe.restype = C.c_int
     return lib
 
@@ -132,11 +132,9 @@ def _scene(kind):
     return S.soft_plane_env(3, 0, nt_range=(2e3, 6e3), geom_order=1)
 
 
-@pytest.mark.parametrize("kind,split", [(KTRI, 0), (KTRI, 3), (KTRI, -3), (KTET, 0), (KSOFT, 0), (KSOFT, 2),
-                                        (KSOFT, -2), (KPLANE, 0)])
+@pytest.mark.parametrize("kind,split", [(KTRI, 0), (KTRI, 3), (KTET, 0), (KSOFT, 0), (KSOFT, 2), (KPLANE, 0)])
 def test_kernel_bodies_match_the_oracle(emu, kind, split):
-    """split > 0: items over LBVH subtrees; split = -q: items of 64 q
-    queries (queries_per_lane q)."""
+    """split > 0: items over LBVH subtrees."""
     sc = _scene(kind)
     W, counts, stats = run_pair(emu, sc, kind, split=split, representation=split % 2)
     ref = oracle(sc, kind, representation=split % 2)
@@ -149,17 +147,16 @@ def test_kernel_bodies_match_the_oracle(emu, kind, split):
         assert stats[1] > 0  # subtree items away from the contact are culled
 
 
-@pytest.mark.parametrize("kind,split,qpl", [(KTRI, 0, 0), (KTRI, 2, 2), (KSOFT, 0, 0), (KSOFT, 2, 3)])
-def test_general_kernel_matches_the_oracle(emu, kind, split, qpl):
+@pytest.mark.parametrize("kind,split", [(KTRI, 0), (KTRI, 2), (KSOFT, 0), (KSOFT, 2)])
+def test_general_kernel_matches_the_oracle(emu, kind, split):
     """The general (not kind-specialised) candidate and clip bodies, which
-    launches over pairs of mixed kinds run; with qpl > 0, items of 64 qpl
-    queries on top of the subtree split."""
+    launches over pairs of mixed kinds run."""
     sc = _scene(kind)
-    emu.emu_set_options(1, qpl)
+    emu.emu_set_options(1)
     try:
         W, counts, stats = run_pair(emu, sc, kind, split=split)
     finally:
-        emu.emu_set_options(0, 0)
+        emu.emu_set_options(0)
     ref = oracle(sc, kind)
     np.testing.assert_array_equal(counts, ref["counts"])
     assert_wrench_close(W[0], ref["wrench"])
