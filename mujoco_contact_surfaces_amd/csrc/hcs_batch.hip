@@ -33,6 +33,10 @@ struct hcs_batch {
    * this into sub-items over LBVH subtrees (0: never) */
   int split_target = 2048;
   int queues = 1; /* candidate-kernel work queues: 1, or one per XCD */
+  /* work order: 0 items by decreasing measured candidate count; 1 pairs by
+   * decreasing total count, each pair's items together (decreasing count),
+   * so a pair's tree and records stay in L2 while its items run */
+  int order_mode = 0;
   bool has_state = false;
   DBuf envs, state, items;
   Pipe pipe;
@@ -125,8 +129,10 @@ static int batch_measure(hcs_batch* b, std::vector<int>& nc) {
   return rc;
 }
 
-/* Work order of the items for the candidate kernel: decreasing measured
- * candidate count (longest first), ties by item index. */
+/* Work order of the items for the candidate kernel (order_mode): decreasing
+ * measured candidate count (longest first), or pairs by decreasing total
+ * count with each pair's items together by decreasing count; ties by item
+ * index. */
 static int batch_order(hcs_batch* b) {
   std::vector<int> nc;
   int rc = batch_measure(b, nc);
@@ -134,7 +140,17 @@ static int batch_order(hcs_batch* b) {
   std::vector<int>& o = b->plan.order;
   o.resize(nc.size());
   for (size_t i = 0; i < o.size(); ++i) o[i] = (int)i;
-  std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return nc[x] > nc[y]; });
+  if (b->order_mode == 1) {
+    std::vector<int64_t> tot(b->npairs, 0);
+    for (size_t i = 0; i < nc.size(); ++i) tot[b->plan.items[i].env] += nc[i];
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) {
+      const int ex = b->plan.items[x].env, ey = b->plan.items[y].env;
+      if (ex != ey) return tot[ex] != tot[ey] ? tot[ex] > tot[ey] : ex < ey;
+      return nc[x] > nc[y];
+    });
+  } else {
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return nc[x] > nc[y]; });
+  }
   return batch_upload(b);
 }
 
@@ -425,14 +441,15 @@ static int time_steps(hcs_batch* b, int reps, double* ms) {
 int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
   if (!b || reps <= 0) return HCS_ERR_ARG;
   /* coordinate descent: (tet-query threshold, queues, blocks per CU) at the
-   * current split target, then the split target at the best of those with
-   * the best blocks per CU and its neighbours */
+   * current split target and work order, then the split target, then the
+   * work order, each at the best so far with the best blocks per CU and its
+   * neighbours */
   const int bpc[] = {2, 4, 8, 12};
   const int queues[] = {1, HCS_QUEUES};
   const int tets[] = {0, 64, 256};
   const int splits[] = {0, 512, 1024, 2048, 4096};
   double best = 1e300;
-  int bb = b->blocks_per_cu, bq = b->queues, bt = b->tet_max, bs = b->split_target;
+  int bb = b->blocks_per_cu, bq = b->queues, bt = b->tet_max, bs = b->split_target, bo = b->order_mode;
   auto try_plan = [&](const int* ks, int nk) -> int {
     int rc = batch_plan(b);
     if (rc) return rc;
@@ -441,7 +458,8 @@ int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
       double ms = 0;
       rc = time_steps(b, reps, &ms);
       if (rc) return rc;
-      if (ms < best) best = ms, bb = ks[i], bq = b->queues, bt = b->tet_max, bs = b->split_target;
+      if (ms < best)
+        best = ms, bb = ks[i], bq = b->queues, bt = b->tet_max, bs = b->split_target, bo = b->order_mode;
     }
     return HCS_OK;
   };
@@ -463,7 +481,13 @@ int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
     int rc = try_plan(near.data(), (int)near.size());
     if (rc) return rc;
   }
-  b->blocks_per_cu = bb, b->queues = bq, b->tet_max = bt, b->split_target = bs;
+  b->split_target = bs;
+  b->order_mode = 1 - bo;
+  {
+    int rc = try_plan(near.data(), (int)near.size());
+    if (rc) return rc;
+  }
+  b->blocks_per_cu = bb, b->queues = bq, b->tet_max = bt, b->split_target = bs, b->order_mode = bo;
   int rc = batch_plan(b);
   if (!rc) rc = batch_capture(b);
   if (best_ms) *best_ms = best;

@@ -152,11 +152,6 @@ int clip_grid_for(const Ctx* c, int64_t cap);
 void launch_prep(hipStream_t s, const EnvDesc* envs, const double* state, double* xf, int n);
 void launch_fill_u32(hipStream_t s, unsigned int* p, unsigned int v, int n);
 
-struct TaxelGridDev {
-  double X[12];
-  double xs, ys, zs, res, sigma;
-  int S, window, cx, cy;
-};
 int taxel_grid_dev(const hcs_taxel_grid& G, TaxelGridDev& g);
 int run_taxels(Ctx* c, const TaxelGridDev& g, const unsigned char* d_mask, const double* poly, const double* ptan,
                const int64_t* key, int64_t np, const double* Xd, double* d_out);
@@ -173,10 +168,11 @@ struct BatchTaxels {
       cell_base /* int[n + 1] */, cell_sensor /* int per taxel */, psoff /* int[npairs + 1] */, psens /* int[n] */,
       mask /* uchar per pair: has a sensor */, poly_rec, poly_key, poly_cnt /* ull */, cnt /* unsigned[ncell + 1] */,
       cursor /* unsigned[ncell] */, off /* unsigned[ncell + 1] */, scan_tmp, list /* int[list_cap] */,
-      out /* double per taxel */;
+      out /* double per taxel */, live /* int[ncell]: taxels with candidate polygons */;
   int64_t poly_cap = 0, list_cap = 0;
   size_t scan_bytes = 0;
   int bin_grid = 1;
+  int max_rays = 0; /* the largest S^2 of the sensors */
 };
 int batch_taxels_setup(Ctx* c, BatchTaxels& t, int npairs, const int32_t* geoms, int n, const int32_t* pair,
                        const hcs_taxel_grid* grids);

@@ -202,4 +202,13 @@ struct ClipArgs {
                   is beyond it (survivor list overflow) writes nothing */
 };
 
+/* A flat sensor's taxel grid on the device: pose (rotation row-major,
+ * position), half sizes, taxel pitch, window sigma, S rays per taxel side,
+ * window kind, taxels along x and y. */
+struct TaxelGridDev {
+  double X[12];
+  double xs, ys, zs, res, sigma;
+  int S, window, cx, cy;
+};
+
 }  // namespace hcs

@@ -94,6 +94,11 @@ __device__ inline void make_item_ctx<Dual>(const EnvDesc& E, const double* st, i
   c.repr = E.repr;
 }
 
+/* Lanes of mask m below this lane. */
+__device__ inline unsigned lane_prefix(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 struct OutPtrs {
   double* face_rec;
   int64_t* face_key;
@@ -127,7 +132,16 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
   rot(ic.pw.X_WA, gA, gAW);
   const int n = p.n;
   if (o && o->poly_rec && (!o->poly_env || o->poly_env[env])) {
-    unsigned long long slot = atomicAdd(o->poly_cnt, 1ull);
+    /* the lanes that keep a polygon here reserve their record slots with
+     * one atomic (the lowest of them adds the count, the others read its
+     * base): a per-polygon atomic on the one counter serialises when a
+     * step keeps millions of polygons */
+    const unsigned long long m = __ballot(1);
+    const int lead = __builtin_ctzll(m);
+    unsigned long long base = 0;
+    if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(o->poly_cnt, (unsigned long long)__popcll(m));
+    base = __shfl(base, lead, 64);
+    const unsigned long long slot = base + lane_prefix(m);
     if ((int64_t)slot < o->poly_cap) {
       double* P = o->poly_rec + slot * HCS_PREC;
       P[0] = n;
@@ -459,9 +473,6 @@ __device__ inline bool box_may_meet_triangle(const double* bx, const double* v, 
   return true;
 }
 
-__device__ inline unsigned lane_prefix(unsigned long long m) {
-  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
 
 template <class T>
 __device__ inline T wave_sum(T v);

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -28,32 +29,13 @@
 #include "hcs_device.h"
 #include "hcs_internal.h"
 #include "hcs_prim.h"
+#include "hcs_taxel_dev.h"
 
 #pragma clang fp contract(off)
 
 using namespace hcs;
 
 namespace hcs {
-
-__device__ inline void footprint(const double* P, const TaxelGridDev& g, int& x0, int& x1, int& y0, int& y1) {
-  const int n = (int)P[0];
-  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
-  for (int i = 0; i < n; ++i) {
-    const double* v = P + 1 + 3 * i;
-    double d[3] = {v[0] - g.X[9], v[1] - g.X[10], v[2] - g.X[11]};
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      double l = (g.X[c] * d[0] + g.X[3 + c] * d[1]) + g.X[6 + c] * d[2];
-      lo[c] = fmin(lo[c], l), hi[c] = fmax(hi[c], l);
-    }
-  }
-  const double m = 1e-9;
-  x0 = (int)floor((lo[0] - m + g.xs) / g.res);
-  x1 = (int)floor((hi[0] + m + g.xs) / g.res);
-  y0 = (int)floor((lo[1] - m + g.ys) / g.res);
-  y1 = (int)floor((hi[1] + m + g.ys) / g.res);
-  x0 = max(x0, 0), y0 = max(y0, 0), x1 = min(x1, g.cx - 1), y1 = min(y1, g.cy - 1);
-}
 
 __global__ void k_tx_bin(const double* __restrict__ poly, const int64_t* __restrict__ key, int64_t np,
                          const unsigned char* __restrict__ env_mask, TaxelGridDev g, unsigned* __restrict__ cnt,
@@ -76,242 +58,8 @@ __global__ void k_tx_bin(const double* __restrict__ poly, const int64_t* __restr
     }
 }
 
-__device__ inline int64_t pkey(const int64_t* key, int q) {
-  return (key[3 * q] << 42) | (key[3 * q + 1] << 21) | key[3 * q + 2];
-}
-
-/* One workgroup per taxel; samples strided over the threads, each thread
- * sums its samples in index order, then a fixed tree reduction.  T = Dual
- * carries one tangent direction: polygon tangents in ptan, sensor pose
- * tangent in Xd (the hit triangle is chosen on primal values). */
 template <class T>
-__device__ inline T ld(const double* v, const double* d, size_t i);
-template <>
-__device__ inline double ld<double>(const double* v, const double*, size_t i) {
-  return v[i];
-}
-template <>
-__device__ inline Dual ld<Dual>(const double* v, const double* d, size_t i) {
-  return Dual(v[i], d[i]);
-}
-
-/* Polygons of one chunk of a taxel's list, staged in LDS for its rays: per
- * fan triangle (v_k, v_k+1, C) the sample-index ranges [i0, i1] x [j0, j1]
- * (i in the low, i1 / j1 in the high 16 bits) outside of which no ray of the
- * taxel can hit it, so a ray runs the Moeller-Trumbore test only on the fans
- * whose range holds it. */
-#define HCS_TX_CHUNK 128
-struct TxStage {
-  uint2 rng[HCS_TX_CHUNK][8];
-  int q[HCS_TX_CHUNK], nv[HCS_TX_CHUNK];
-  int64_t kq[HCS_TX_CHUNK];
-};
-
-/* The sample-index ranges of the fans of polygon P for taxel (x, y).
- *  - |a| < 1e-10 (a = e1 . (D x e2), the same expression the hit test
- *    rejects on): no ray can hit; empty range.
- *  - |a| < 1e-3 max(|e1|, |e2|)^2 (nearly edge-on to the rays): every ray.
- *  - otherwise the rays whose sensor-plane position lies within m of the
- *    fan's projected box, m = 1e-9 (1 + extent + height gap).  For such a
- *    triangle the computed barycentrics err by less than ~7e-13 (L + |s|) / L
- *    (L the longest edge, s = origin - v0), so an accepted ray lies within
- *    ~7e-13 (L + |s|) of the projected triangle: m is three orders wider, and
- *    the floor / ceil below add up to one more sample on each side. */
-__device__ inline void stage_fans(const double* P, const TaxelGridDev& g, int x, int y, uint2* rng) {
-  const int S = g.S;
-  const double rSR = g.res / S;
-  const int nv = min((int)P[0], 8);
-  const double Dv[3] = {-g.X[2], -g.X[5], -g.X[8]};
-  double px[9], py[9], pz[9];
-  for (int k = 0; k <= nv && k < 9; ++k) {
-    const double* v = k < nv ? P + 1 + 3 * k : P + 33;
-    const double d[3] = {v[0] - g.X[9], v[1] - g.X[10], v[2] - g.X[11]};
-    const int o = k < nv ? k : 8;
-    px[o] = (g.X[0] * d[0] + g.X[3] * d[1]) + g.X[6] * d[2];
-    py[o] = (g.X[1] * d[0] + g.X[4] * d[1]) + g.X[7] * d[2];
-    pz[o] = (g.X[2] * d[0] + g.X[5] * d[1]) + g.X[8] * d[2];
-  }
-  const double ox = -g.xs + x * g.res + 0.5 * rSR, oy = -g.ys + y * g.res + 0.5 * rSR, oz = 1.5 * g.zs;
-  for (int k = 0; k < nv; ++k) {
-    const int k1 = (k + 1 == nv) ? 0 : k + 1;
-    const double* v0 = P + 1 + 3 * k;
-    const double* v1 = P + 1 + 3 * k1;
-    const double* v2 = P + 33;
-    double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
-    double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
-    double h[3];
-    cross3(Dv, e2, h);
-    const double a = dot3(e1, h);
-    const double L2 = fmax(dot3(e1, e1), dot3(e2, e2));
-    uint2 r;
-    if (fabs(a) < 1e-10) {
-      r = make_uint2(1u, 1u); /* i0 = 1 > i1 = 0: empty */
-    } else if (fabs(a) < 1e-3 * L2) {
-      r = make_uint2((unsigned)(S - 1) << 16, (unsigned)(S - 1) << 16);
-    } else {
-      const int id[3] = {k, k1, 8};
-      double lx = 1e300, hx = -1e300, ly = 1e300, hy = -1e300, gap = 0.0;
-#pragma unroll
-      for (int t = 0; t < 3; ++t) {
-        lx = fmin(lx, px[id[t]]), hx = fmax(hx, px[id[t]]);
-        ly = fmin(ly, py[id[t]]), hy = fmax(hy, py[id[t]]);
-        gap = fmax(gap, fabs(oz - pz[id[t]]));
-      }
-      const double m = 1e-9 * (1.0 + fmax(hx - lx, hy - ly) + gap);
-      /* clamp in double before the conversions */
-      const double fi0 = fmax(floor((lx - m - ox) / rSR), 0.0), fi1 = fmin(ceil((hx + m - ox) / rSR), S - 1.0);
-      const double fj0 = fmax(floor((ly - m - oy) / rSR), 0.0), fj1 = fmin(ceil((hy + m - oy) / rSR), S - 1.0);
-      if (!(fi0 <= fi1) || !(fj0 <= fj1)) {
-        r = make_uint2(1u, 1u);
-      } else {
-        r = make_uint2((unsigned)fi0 | ((unsigned)fi1 << 16), (unsigned)fj0 | ((unsigned)fj1 << 16));
-      }
-    }
-    rng[k] = r;
-  }
-}
-
-/* The value of taxel (x, y): the block's 256 threads sample it; the block's
- * candidate polygons are list[base .. base + n). */
-template <class T>
-__device__ inline double sample_cell(const double* __restrict__ poly, const double* __restrict__ ptan,
-                                     const int64_t* __restrict__ key, const TaxelGridDev& g,
-                                     const double* __restrict__ Xd, unsigned base, unsigned n,
-                                     const int* __restrict__ list, int x, int y) {
-  __shared__ TxStage L;
-  const int S = g.S;
-  const double rSR = g.res / S;
-  const double rmean = 1. / (S * S);
-  const double di_factor = g.res / S;
-  const double sub_halfwidth = di_factor / 2.0 - g.res / 2.0;
-  const double max_dist = 1.41421356237 * g.res / 2.0;
-  T X[12];
-#pragma unroll
-  for (int i = 0; i < 12; ++i) X[i] = ld<T>(g.X, Xd, i);
-  const T nrm[3] = {X[2], X[5], X[8]};
-  double rsigma_squared = 0, rtukey = 0;
-  if (g.window == 1) rsigma_squared = 0.5 / (g.sigma * g.sigma);
-  if (g.window == 2) rtukey = 1. / g.sigma * S * S;
-  const double PI = 3.14159265358979323846;
-  T acc = T(0.0);
-  /* rays in rounds of blockDim.x (every thread runs every round, so the
-   * staging barriers are reached by all) */
-  for (int s0 = 0; s0 < S * S; s0 += blockDim.x) {
-    const int s = s0 + (int)threadIdx.x;
-    const bool active = s < S * S;
-    const int i = active ? s / S : 0, j = active ? s % S : 0;
-    T loc[3] = {T(-g.xs + x * g.res + i * rSR + 0.5 * rSR), T(-g.ys + y * g.res + j * rSR + 0.5 * rSR),
-                T(1.5 * g.zs)};
-    T pt[3];
-    xform(X, loc, pt);
-    T O[3], D[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) O[c] = pt[c] + nrm[c] * 1e-8, D[c] = -nrm[c];
-    double best_t = 1e30;
-    int64_t bkey = INT64_MAX;
-    int bk = 0, bq = -1;
-    const double Ov[3] = {val(O[0]), val(O[1]), val(O[2])}, Dv[3] = {val(D[0]), val(D[1]), val(D[2])};
-    for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
-      const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
-      __syncthreads();
-      for (unsigned l = threadIdx.x; l < cn; l += blockDim.x) {
-        const int q = list[base + c0 + l];
-        const double* P = poly + (size_t)q * HCS_PREC;
-        L.q[l] = q;
-        L.nv[l] = min((int)P[0], 8);
-        L.kq[l] = pkey(key, q);
-        stage_fans(P, g, x, y, L.rng[l]);
-      }
-      __syncthreads();
-      if (!active) continue;
-      for (unsigned l = 0; l < cn; ++l) {
-        const int nv = L.nv[l];
-        const int q = L.q[l];
-        const int64_t kq = L.kq[l];
-        const double* P = poly + (size_t)q * HCS_PREC;
-        for (int k = 0; k < nv; ++k) {
-          const uint2 r = L.rng[l][k];
-          if (i < (int)(r.x & 0xffffu) || i > (int)(r.x >> 16) || j < (int)(r.y & 0xffffu) || j > (int)(r.y >> 16))
-            continue;
-          const int k1 = (k + 1 == nv) ? 0 : k + 1;
-          const double* v0 = P + 1 + 3 * k;
-          const double* v1 = P + 1 + 3 * k1;
-          const double* v2 = P + 33;
-          double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
-          double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
-          double h[3];
-          cross3(Dv, e2, h);
-          const double a = dot3(e1, h);
-          if (fabs(a) < 1e-10) continue;
-          const double f = 1.0 / a;
-          double sv[3] = {Ov[0] - v0[0], Ov[1] - v0[1], Ov[2] - v0[2]};
-          const double u = f * dot3(sv, h);
-          if (u < 0.0 || u > 1.0) continue;
-          double qv[3];
-          cross3(sv, e1, qv);
-          const double v = f * dot3(Dv, qv);
-          if (v < 0.0 || u + v > 1.0) continue;
-          const double t = f * dot3(e2, qv);
-          if (!(t > 0.0)) continue;
-          const bool better = t < best_t || (t == best_t && (kq < bkey || (kq == bkey && k < bk)));
-          if (better) best_t = t, bkey = kq, bk = k, bq = q;
-        }
-      }
-    }
-    if (!active) continue;
-    if (bq >= 0 && best_t < 1.5 * g.zs && best_t > 0.0) {
-      /* Moeller-Trumbore on the chosen triangle, in T arithmetic */
-      const size_t pb = (size_t)bq * HCS_PREC;
-      const int nv = (int)poly[pb];
-      const int k1 = (bk + 1 == nv) ? 0 : bk + 1;
-      T v0[3], v1[3], v2[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        v0[c] = ld<T>(poly, ptan, pb + 1 + 3 * bk + c);
-        v1[c] = ld<T>(poly, ptan, pb + 1 + 3 * k1 + c);
-        v2[c] = ld<T>(poly, ptan, pb + 33 + c);
-      }
-      T e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
-      T e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
-      T h[3];
-      cross3(D, e2, h);
-      const T f = T(1.0) / dot3(e1, h);
-      T sv[3] = {O[0] - v0[0], O[1] - v0[1], O[2] - v0[2]};
-      const T u = f * dot3(sv, h);
-      T qv[3];
-      cross3(sv, e1, qv);
-      const T v = f * dot3(D, qv);
-      const T pa = ld<T>(poly, ptan, pb + 25 + bk), pbb = ld<T>(poly, ptan, pb + 25 + k1),
-              pc = ld<T>(poly, ptan, pb + 36);
-      const T b0 = T(1.0) - u - v;
-      const T raw = ((b0 * pa + u * pbb) + v * pc) * rmean;
-      double weight = 1.0;
-      if (g.window == 1) {
-        const double dist = hypot(di_factor * i + sub_halfwidth, di_factor * j + sub_halfwidth) / max_dist;
-        weight = exp(-(dist * dist) * rsigma_squared);
-      } else if (g.window == 2) {
-        const int ni = S / 2 - abs(S / 2 - i), nj = S / 2 - abs(S / 2 - j);
-        if (ni <= g.sigma * S / 2) weight *= 0.5 * (1. - cos(2. * PI * ni * rtukey));
-        if (nj <= g.sigma * S / 2) weight *= 0.5 * (1. - cos(2. * PI * nj * rtukey));
-      } else if (g.window == 3) {
-        const double inv = 1 - hypot(di_factor * i + sub_halfwidth, di_factor * j + sub_halfwidth) / max_dist;
-        weight = inv * inv;
-      }
-      acc = acc + weight * raw;
-    }
-  }
-  __shared__ double red[256];
-  red[threadIdx.x] = ptan ? tangent(acc) : val(acc);
-  __syncthreads();
-  for (int w = 128; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
-    __syncthreads();
-  }
-  return red[0];
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_tx_sample(const double* __restrict__ poly, const double* __restrict__ ptan,
+__global__ __launch_bounds__(64) void k_tx_sample(const double* __restrict__ poly, const double* __restrict__ ptan,
                                                    const int64_t* __restrict__ key, TaxelGridDev g,
                                                    const double* __restrict__ Xd, const unsigned* __restrict__ off,
                                                    const unsigned* __restrict__ cnt, const int* __restrict__ list,
@@ -319,6 +67,17 @@ __global__ __launch_bounds__(256) void k_tx_sample(const double* __restrict__ po
   const int cell = blockIdx.x;
   const int x = cell % g.cx, y = cell / g.cx;
   const double v = sample_cell<T>(poly, ptan, key, g, Xd, off[cell], cnt[cell], list, x, y);
+  if (threadIdx.x == 0) out[x + g.cx * y] = v;
+}
+
+/* The same, fan-major (double, S^2 <= HCS_TX_RAYS). */
+__global__ __launch_bounds__(64) void k_tx_sample_fan(const double* __restrict__ poly, const int64_t* __restrict__ key,
+                                                      TaxelGridDev g, const unsigned* __restrict__ off,
+                                                      const unsigned* __restrict__ cnt, const int* __restrict__ list,
+                                                      double* __restrict__ out) {
+  const int cell = blockIdx.x;
+  const int x = cell % g.cx, y = cell / g.cx;
+  const double v = sample_cell_scatter(poly, key, g, off[cell], cnt[cell], list, x, y);
   if (threadIdx.x == 0) out[x + g.cx * y] = v;
 }
 
@@ -355,10 +114,13 @@ int run_taxels(Ctx* c, const TaxelGridDev& g, const unsigned char* d_mask, const
     hipLaunchKernelGGL(k_tx_bin, dim3((unsigned)nb), dim3(256), 0, c->stream, poly, key, np, d_mask, g, cnt,
                        (const unsigned*)off, cursor, (int*)c->d_tx_list.ptr);
   if (ptan)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tx_sample<Dual>), dim3(ncell), dim3(256), 0, c->stream, poly, ptan, key, g,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tx_sample<Dual>), dim3(ncell), dim3(64), 0, c->stream, poly, ptan, key, g,
                        Xd, (const unsigned*)off, (const unsigned*)cnt, (const int*)c->d_tx_list.ptr, d_out);
+  else if (g.S * g.S <= HCS_TX_RAYS && c->last_pairs.size() < (1u << 19))
+    hipLaunchKernelGGL(k_tx_sample_fan, dim3(ncell), dim3(64), 0, c->stream, poly, key, g, (const unsigned*)off,
+                       (const unsigned*)cnt, (const int*)c->d_tx_list.ptr, d_out);
   else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tx_sample<double>), dim3(ncell), dim3(256), 0, c->stream, poly,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_tx_sample<double>), dim3(ncell), dim3(64), 0, c->stream, poly,
                        (const double*)nullptr, key, g, (const double*)nullptr, (const unsigned*)off,
                        (const unsigned*)cnt, (const int*)c->d_tx_list.ptr, d_out);
   return hip_check(c, hipGetLastError(), "taxel kernels");
@@ -423,26 +185,44 @@ __global__ __launch_bounds__(256) void k_txb_bin(const double* __restrict__ poly
   }
 }
 
-/* One workgroup per taxel of any sensor. */
-__global__ __launch_bounds__(256) void k_txb_sample(const double* __restrict__ poly, const int64_t* __restrict__ key,
-                                                    const TaxelGridDev* __restrict__ grids,
-                                                    const int* __restrict__ cell_base,
-                                                    const int* __restrict__ cell_sensor,
-                                                    const unsigned* __restrict__ off,
-                                                    const unsigned* __restrict__ cnt, const int* __restrict__ list,
-                                                    int64_t list_cap, double* __restrict__ out) {
-  const int cell = blockIdx.x;
-  const int s = cell_sensor[cell];
-  const unsigned n = cnt[cell], base = off[cell];
-  /* an overflowed list is re-run after the step with a larger buffer */
-  if (n == 0 || (int64_t)base + n > list_cap) {
-    if (threadIdx.x == 0) out[cell] = 0.0;
-    return;
+/* The taxels that have candidate polygons (and whose list fits the list
+ * buffer; an overflowed list is re-run after the step with a larger one),
+ * appended to live[] with one atomic per wave; every other taxel is 0 (the
+ * output is cleared before). */
+__global__ __launch_bounds__(256) void k_txb_live(const unsigned* __restrict__ cnt, const unsigned* __restrict__ off,
+                                                  int ncell, int64_t list_cap, int* __restrict__ live,
+                                                  unsigned* __restrict__ nlive) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = cell < ncell && cnt[cell] > 0 && (int64_t)off[cell] + cnt[cell] <= list_cap;
+  const unsigned long long m = __ballot(on);
+  const int lane = threadIdx.x & 63;
+  unsigned base = 0;
+  if (lane == 0 && m) base = atomicAdd(nlive, (unsigned)__popcll(m));
+  base = __shfl(base, 0, 64);
+  if (on) live[base + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = cell;
+}
+
+/* One wave per live taxel of any sensor, strided over the grid; FAN: the
+ * fan-major sampler (every sensor has S^2 <= HCS_TX_RAYS). */
+template <bool FAN>
+__global__ __launch_bounds__(64) void k_txb_sample(const double* __restrict__ poly, const int64_t* __restrict__ key,
+                                                   const TaxelGridDev* __restrict__ grids,
+                                                   const int* __restrict__ cell_base, const int* __restrict__ cell_sensor,
+                                                   const unsigned* __restrict__ off, const unsigned* __restrict__ cnt,
+                                                   const int* __restrict__ list, const int* __restrict__ live,
+                                                   const unsigned* __restrict__ nlive, double* __restrict__ out) {
+  const unsigned nl = *nlive;
+  for (unsigned k = blockIdx.x; k < nl; k += gridDim.x) {
+    const int cell = live[k];
+    const int s = cell_sensor[cell];
+    const TaxelGridDev g = grids[s];
+    const int local = cell - cell_base[s];
+    const double v =
+        FAN ? sample_cell_scatter(poly, key, g, off[cell], cnt[cell], list, local % g.cx, local / g.cx)
+            : sample_cell<double>(poly, nullptr, key, g, nullptr, off[cell], cnt[cell], list, local % g.cx,
+                                  local / g.cx);
+    if (threadIdx.x == 0) out[cell] = v;
   }
-  const TaxelGridDev g = grids[s];
-  const int local = cell - cell_base[s];
-  const double v = sample_cell<double>(poly, nullptr, key, g, nullptr, base, n, list, local % g.cx, local / g.cx);
-  if (threadIdx.x == 0) out[cell] = v;
 }
 
 int batch_taxels_setup(Ctx* c, BatchTaxels& t, int npairs, const int32_t* geoms, int n, const int32_t* pair,
@@ -484,7 +264,8 @@ int batch_taxels_setup(Ctx* c, BatchTaxels& t, int npairs, const int32_t* geoms,
       (rc = ensure(c, t.psens, sizeof(int) * psens.size())) || (rc = ensure(c, t.mask, mask.size())) ||
       (rc = ensure(c, t.poly_cnt, 64)) || (rc = ensure(c, t.cnt, sizeof(unsigned) * (ncell + 1))) ||
       (rc = ensure(c, t.cursor, sizeof(unsigned) * std::max(1, ncell))) ||
-      (rc = ensure(c, t.off, sizeof(unsigned) * (ncell + 1))) || (rc = ensure(c, t.out, sizeof(double) * std::max(1, ncell))))
+      (rc = ensure(c, t.off, sizeof(unsigned) * (ncell + 1))) || (rc = ensure(c, t.out, sizeof(double) * std::max(1, ncell))) ||
+      (rc = ensure(c, t.live, sizeof(int) * std::max(1, ncell))))
     return rc;
   const size_t sb = scan_temp_bytes((long long)ncell + 1);
   if ((rc = ensure(c, t.scan_tmp, std::max<size_t>(sb, 8)))) return rc;
@@ -501,6 +282,10 @@ int batch_taxels_setup(Ctx* c, BatchTaxels& t, int npairs, const int32_t* geoms,
     if ((rc = hip_check(c, hipMemcpyAsync(u.d.ptr, u.h, u.bytes, hipMemcpyHostToDevice, c->stream), "upload sensors")))
       return rc;
   t.n = n, t.ncell = ncell;
+  t.max_rays = 0;
+  for (int k = 0; k < n; ++k) t.max_rays = std::max(t.max_rays, g[k].S * g[k].S);
+  /* the fan-major sampler packs pair indices into 19 bits */
+  if (npairs >= (1 << 19)) t.max_rays = INT_MAX;
   t.bin_grid = std::max(1, c->num_cu * 4);
   return hip_check(c, hipStreamSynchronize(c->stream), "sync");
 }
@@ -536,11 +321,22 @@ void batch_taxels_enqueue(Ctx* c, BatchTaxels& t, const double* state) {
                      (const int64_t*)t.poly_key.ptr, pc, t.poly_cap, (const int*)t.psoff.ptr, (const int*)t.psens.ptr,
                      (const TaxelGridDev*)g, (const int*)t.cell_base.ptr, (unsigned*)t.cnt.ptr,
                      (const unsigned*)t.off.ptr, (unsigned*)t.cursor.ptr, (int*)t.list.ptr, t.list_cap);
-  if (t.ncell > 0)
-    hipLaunchKernelGGL(k_txb_sample, dim3(t.ncell), dim3(256), 0, s, (const double*)t.poly_rec.ptr,
+  unsigned* nlive = (unsigned*)t.poly_cnt.ptr + 2;
+  launch_fill_u32(s, nlive, 0u, 1);
+  launch_fill_u32(s, (unsigned*)t.out.ptr, 0u, 2 * t.ncell);
+  hipLaunchKernelGGL(k_txb_live, dim3((t.ncell + 255) / 256), dim3(256), 0, s, (const unsigned*)t.cnt.ptr,
+                     (const unsigned*)t.off.ptr, t.ncell, t.list_cap, (int*)t.live.ptr, nlive);
+  const dim3 grid(std::max(1, std::min(t.ncell, c->num_cu * 16)));
+  if (t.max_rays <= HCS_TX_RAYS)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_txb_sample<true>), grid, dim3(64), 0, s, (const double*)t.poly_rec.ptr,
                        (const int64_t*)t.poly_key.ptr, (const TaxelGridDev*)g, (const int*)t.cell_base.ptr,
                        (const int*)t.cell_sensor.ptr, (const unsigned*)t.off.ptr, (const unsigned*)t.cnt.ptr,
-                       (const int*)t.list.ptr, t.list_cap, (double*)t.out.ptr);
+                       (const int*)t.list.ptr, (const int*)t.live.ptr, (const unsigned*)nlive, (double*)t.out.ptr);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_txb_sample<false>), grid, dim3(64), 0, s, (const double*)t.poly_rec.ptr,
+                       (const int64_t*)t.poly_key.ptr, (const TaxelGridDev*)g, (const int*)t.cell_base.ptr,
+                       (const int*)t.cell_sensor.ptr, (const unsigned*)t.off.ptr, (const unsigned*)t.cnt.ptr,
+                       (const int*)t.list.ptr, (const int*)t.live.ptr, (const unsigned*)nlive, (double*)t.out.ptr);
 }
 
 int batch_taxels_demand(Ctx* c, BatchTaxels& t, int64_t& npoly, int64_t& nlist) {
@@ -559,7 +355,7 @@ int batch_taxels_demand(Ctx* c, BatchTaxels& t, int64_t& npoly, int64_t& nlist) 
 
 void batch_taxels_free(BatchTaxels& t) {
   for (DBuf* b : {&t.grid, &t.src, &t.cell_base, &t.cell_sensor, &t.psoff, &t.psens, &t.mask, &t.poly_rec, &t.poly_key,
-                  &t.poly_cnt, &t.cnt, &t.cursor, &t.off, &t.scan_tmp, &t.list, &t.out})
+                  &t.poly_cnt, &t.cnt, &t.cursor, &t.off, &t.scan_tmp, &t.list, &t.out, &t.live})
     if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
   t = BatchTaxels();
 }

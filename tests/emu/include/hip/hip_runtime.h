@@ -22,6 +22,8 @@
 #define __host__
 #define __global__
 #define __forceinline__ inline
+/* one wave at a time: a function-local static is the wave's LDS */
+#define __shared__ static
 
 struct int2 {
   int x, y;
@@ -31,11 +33,16 @@ struct int4 {
   int x, y, z, w;
 };
 inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
+struct uint2 {
+  unsigned x, y;
+};
+inline uint2 make_uint2(unsigned a, unsigned b) { return uint2{a, b}; }
 struct float4 {
   float x, y, z, w;
 };
 inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
 inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
 
 struct Dim3 {
   unsigned x, y, z;
@@ -108,6 +115,13 @@ inline unsigned long long __shfl(unsigned long long v, int src, int = 64, const 
   g_wave->bar.arrive_and_wait();
   return r;
 }
+inline double __shfl(double v, int src, int = 64, const emu_site& at = emu_site::current()) {
+  g_wave->d[g_lane] = v;
+  emu_meet(at);
+  double r = g_wave->d[src];
+  g_wave->bar.arrive_and_wait();
+  return r;
+}
 inline unsigned __shfl(unsigned v, int src, int w = 64, const emu_site& at = emu_site::current()) {
   return (unsigned)__shfl((unsigned long long)v, src, w, at);
 }
@@ -126,3 +140,19 @@ inline unsigned long long atomicAdd(unsigned long long* p, unsigned long long v)
 }
 inline unsigned atomicAdd(unsigned* p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v) {
+  unsigned long long o = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (v < o && !__atomic_compare_exchange_n(p, &o, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+  return o;
+}
+inline long long __double_as_longlong(double x) {
+  long long r;
+  std::memcpy(&r, &x, 8);
+  return r;
+}
+inline double __longlong_as_double(long long x) {
+  double r;
+  std::memcpy(&r, &x, 8);
+  return r;
+}

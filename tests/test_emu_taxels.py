@@ -1,0 +1,84 @@
+"""The flat-sensor samplers of hcs_taxel_dev.h built for the CPU (one
+emulated 64-lane wave per taxel, tests/emu/emu_taxels.cpp): the fan-major
+sampler picks the same triangle for every ray as the ray-major one, so the
+taxels are bit-identical, and both match the oracle's sampling (orc_taxels)
+within 1e-10."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from mujoco_contact_surfaces_amd import scenes as S
+from meshes import rot
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("emutx") / "libemu_tx.so")
+    cmd = ["g++", "-O2", "-std=c++20", "-ffp-contract=off", "-fPIC", "-shared", "-Wno-unknown-pragmas",
+           "-I", os.path.join(ROOT, "tests", "emu", "include"),
+           "-I", os.path.join(ROOT, "mujoco_contact_surfaces_amd", "csrc"),
+           os.path.join(ROOT, "tests", "emu", "emu_taxels.cpp"), "-o", out, "-lpthread"]
+    subprocess.run(cmd, check=True, timeout=600)
+    lib = C.CDLL(out)
+    lib.emu_taxels.restype = C.c_int
+    lib.emu_taxels.argtypes = [_D, C.c_int64, _I64, _D, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, _D]
+    return lib
+
+
+def _moved(sc, k):
+    """The scene under a rigid motion of the whole world."""
+    rng = np.random.default_rng(7 + k)
+    ax = rng.normal(size=3)
+    RT, pT = rot(ax / np.linalg.norm(ax), rng.uniform(0.2, 1.0)), rng.uniform(-0.1, 0.1, 3)
+    out = dict(sc)
+    for n in ("X_WA", "X_WB"):
+        R, p = sc[n][:9].reshape(3, 3), sc[n][9:]
+        out[n] = np.concatenate([(RT @ R).reshape(9), RT @ p + pT])
+    return out
+
+
+def _polys(sc):
+    s, r = sc["soft"], sc["rigid"]
+    ref = O.soft_rigid(s["V"], s["T"], s["p"], r["V"], r["F"], sc["X_WA"], sc["X_WB"], sc["V_WA"], sc["V_WB"],
+                       s["props"]["hydroelastic_modulus"], s["props"]["dissipation"], s["props"]["dynamic_friction"],
+                       r["props"]["dynamic_friction"])
+    P = np.ascontiguousarray(ref["polys"], np.float64)
+    K = np.zeros((P.shape[0], 3), np.int64)
+    K[:, 1:] = ref["poly_keys"]
+    return P, K
+
+
+def _run(emu, P, K, X, g, S_, window, sigma, mode):
+    cx = int(np.floor(2 * g["half_size"][0] / g["resolution"] + 0.1))
+    cy = int(np.floor(2 * g["half_size"][1] / g["resolution"] + 0.1))
+    sensor = np.ascontiguousarray(np.concatenate([X, g["half_size"], [g["resolution"]]]), np.float64)
+    out = np.zeros(cx * cy)
+    rc = emu.emu_taxels(P.ctypes.data_as(_D), P.shape[0], K.ctypes.data_as(_I64), sensor.ctypes.data_as(_D), S_,
+                        window, sigma, cx, cy, mode, out.ctypes.data_as(_D))
+    assert rc == 0
+    return out.reshape(cy, cx)
+
+
+@pytest.mark.parametrize("case,window,sigma", [("pad4", 0, 0.0), ("pad4", 1, 0.3), ("pad4", 2, 0.5), ("pad4", 3, 0.0),
+                                               ("pad4_moved", 0, 0.0), ("pad2", 0, 0.0)])
+def test_fan_major_sampler_equals_ray_major_and_oracle(emu, case, window, sigma):
+    sc = S.differentiation_env(0, 1) if case != "pad2" else S.tactile_pad(0)
+    if case.endswith("moved"):
+        sc = _moved(sc, 3)
+    P, K = _polys(sc)
+    g = sc["grid"]
+    S_ = g["sampling_resolution"]
+    a = _run(emu, P, K, sc["X_WB"], g, S_, window, sigma, 0)
+    b = _run(emu, P, K, sc["X_WB"], g, S_, window, sigma, 1)
+    ref = O.taxels(P, sc["X_WB"], g["half_size"], g["resolution"], S_, window, sigma)
+    assert np.abs(ref).max() > 0
+    np.testing.assert_array_equal(b, a)
+    np.testing.assert_allclose(b, ref, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
