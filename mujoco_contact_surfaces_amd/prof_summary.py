@@ -20,7 +20,9 @@ prints, for the last dispatch of every kernel matching --kernels:
   * occupancy: SQ_WAVE_CYCLES / GRBM_GUI_ACTIVE / CUs (mean resident waves per
     CU; on gfx950 SQ_WAVE_CYCLES counts in units of 4 cycles, the factor the
     OccupancyPercent expression in rocprofiler-sdk's counter_defs.yaml uses),
-    and SQ_WAVES (waves launched);
+    and SQ_WAVES (waves launched).  GRBM_GUI_ACTIVE comes summed over the
+    accelerator complex dies (XCDs), so it is divided by their number
+    (Num_Xcc of the run's agent_info.csv) to give the dispatch's clocks;
   * vector-memory path: TD_TD_BUSY_sum and TA_TA_BUSY_sum over GRBM_GUI_ACTIVE
     and CUs (busy fraction of the texture data / address units), the mean L2
     read latency TCP_TCC_READ_REQ_LATENCY_sum / TCP_TCC_READ_REQ_sum (cycles).
@@ -112,6 +114,14 @@ def _last_dispatch_values(d, kernel_re):
     return out
 
 
+def _num_xcc(dirs):
+    for d in dirs:
+        for r in _rows(d, "agent_info.csv"):
+            if r.get("Agent_Type") == "GPU" and r.get("Num_Xcc"):
+                return int(r["Num_Xcc"])
+    return 1
+
+
 def _copy_counter(d, counter):
     rows = _rows(d, "counter_collection.csv")
     vals = [float(_col(r, "Counter_Value")) for r in rows
@@ -154,6 +164,7 @@ def counters(args):
     lines.append("| kernel | " + " | ".join(c for c in cols) + " |")
     lines.append("|---|" + "---|" * len(cols))
     derived = OrderedDict()
+    xcc = args.xcc or _num_xcc([d for d in [args.fetch, args.write] + (args.passes or []) if d])
     for k, v in table.items():
         lines.append("| `%s` | " % _short(k, 50) + " | ".join("%.0f" % v[c] if c in v else "" for c in cols) + " |")
         dv = OrderedDict()
@@ -165,6 +176,8 @@ def counters(args):
             dv["hbm_bytes"] = dv["hbm_read_bytes"] + dv["hbm_write_bytes"]
         g = v.get("GRBM_GUI_ACTIVE")
         if g:
+            g = g / xcc
+            dv["gpu_clocks"] = g
             if "SQ_WAVE_CYCLES" in v:
                 dv["mean_waves_per_cu"] = 4.0 * v["SQ_WAVE_CYCLES"] / g / args.cus
                 dv["occupancy_pct_of_32"] = 100.0 * dv["mean_waves_per_cu"] / 32.0
@@ -177,7 +190,8 @@ def counters(args):
         if "SQ_WAVES" in v:
             dv["waves"] = v["SQ_WAVES"]
         derived[k] = dv
-    lines += ["", "Derived (GRBM_GUI_ACTIVE is the dispatch's duration in GPU clocks; %d CUs):" % args.cus, "",
+    lines += ["", "Derived (GRBM_GUI_ACTIVE / %d XCDs is the dispatch's duration in GPU clocks; %d CUs):" % (xcc, args.cus),
+              "",
               "| kernel | HBM read GB | HBM write GB | HBM GB | mean waves / CU | occupancy % of 32 | TD busy % | "
               "TA busy % | L2 read latency (cycles) |", "|---|---|---|---|---|---|---|---|---|"]
 
@@ -198,7 +212,7 @@ def counters(args):
             m = re.search(r"(k_[a-z_0-9]+)", k)
             short[m.group(1) if m else k] = dv
         with open(args.json, "w") as fh:
-            json.dump(dict(source=args.source, workload=args.workload, per_kernel=short,
+            json.dump(dict(source=args.source, workload=args.workload, num_xcc=xcc, per_kernel=short,
                            hbm_bytes_per_launch={k: v["hbm_bytes"] for k, v in short.items() if "hbm_bytes" in v}),
                       fh, indent=1)
 
@@ -219,6 +233,7 @@ def main(argv=None):
     c.add_argument("--calib-mib", type=int, default=1024)
     c.add_argument("--pass", dest="passes", action="append")
     c.add_argument("--cus", type=int, default=256)
+    c.add_argument("--xcc", type=int, default=0, help="XCDs (default: Num_Xcc of the runs' agent_info.csv, else 1)")
     c.add_argument("--json")
     c.add_argument("--source", default="")
     c.add_argument("--workload", default="")
