@@ -291,7 +291,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the sphere-box, tactile-pad and soft-soft lines")
-    ap.add_argument("--config", default="", help="fixed launch shape: blocks_per_cu,queries_per_lane,tet_max[,split_target[,queues]]")
+    ap.add_argument("--config", default="", help="fixed launch shape: blocks_per_cu,queries_per_lane,tet_max[,split_target[,queues[,order]]]")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (scale reference for the HBM "
                          "byte counters of a profiler) and exit")

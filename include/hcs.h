@@ -256,7 +256,10 @@ int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
  * (must be 1: larger values are rejected), largest rigid mesh handled in
  * tet-query mode, candidate count above which
  * a work item is split over LBVH subtrees (0: never), work queues of the
- * candidate kernel (1, or up to 8: one per XCD)]; 5 int32. */
+ * candidate kernel (1, or up to 8: one per XCD), work order of the items
+ * (0: decreasing candidate count; 1: pairs by decreasing total count, each
+ * pair's items together; 2: decreasing shader cycles of one measured
+ * step)]; 6 int32. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
 int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
 /* Per-replay step times: reps graph replays back to back on hcs_stream with

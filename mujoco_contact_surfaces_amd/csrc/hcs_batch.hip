@@ -35,7 +35,9 @@ struct hcs_batch {
   int queues = 1; /* candidate-kernel work queues: 1, or one per XCD */
   /* work order: 0 items by decreasing measured candidate count; 1 pairs by
    * decreasing total count, each pair's items together (decreasing count),
-   * so a pair's tree and records stay in L2 while its items run */
+   * so a pair's tree and records stay in L2 while its items run; 2 items by
+   * decreasing shader cycles measured in one step (the cost the tail of
+   * the persistent launch depends on) */
   int order_mode = 0;
   bool has_state = false;
   DBuf envs, state, items;
@@ -129,14 +131,42 @@ static int batch_measure(hcs_batch* b, std::vector<int>& nc) {
   return rc;
 }
 
+/* Shader cycles of each item in one eager step on the current state (0 for
+ * items the step culls). */
+static int batch_measure_cycles(hcs_batch* b, std::vector<unsigned long long>& cyc) {
+  Ctx* c = b->c;
+  const int n = (int)b->plan.items.size();
+  cyc.assign(n, 0);
+  if (n == 0) return HCS_OK;
+  DBuf d;
+  const size_t bytes = sizeof(unsigned long long) * 6 * (size_t)n;
+  int rc = ensure(c, d, bytes);
+  if (rc) return rc;
+  launch_fill_u32(c->stream, (unsigned*)d.ptr, 0u, (int)(bytes / 4));
+  QueryArgs a = batch_args(b);
+  a.item_stats = (unsigned long long*)d.ptr;
+  pipe_front(c, b->pipe, a, b->npairs);
+  launch_cand(c->stream, a, grid_of(b), CAND_STEP);
+  std::vector<unsigned long long> h(6 * (size_t)n);
+  rc = hip_check(c, hipGetLastError(), "cycles step");
+  if (!rc) rc = hip_check(c, hipMemcpyAsync(h.data(), d.ptr, bytes, hipMemcpyDeviceToHost, c->stream), "dl");
+  if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
+  (void)hipFree(d.ptr);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) cyc[i] = h[6 * (size_t)i];
+  return HCS_OK;
+}
+
 /* Work order of the items for the candidate kernel (order_mode): decreasing
- * measured candidate count (longest first), or pairs by decreasing total
- * count with each pair's items together by decreasing count; ties by item
- * index. */
+ * measured candidate count (longest first); pairs by decreasing total count
+ * with each pair's items together by decreasing count; or decreasing
+ * measured shader cycles.  Ties by item index. */
 static int batch_order(hcs_batch* b) {
   std::vector<int> nc;
   int rc = batch_measure(b, nc);
   if (rc) return rc;
+  std::vector<unsigned long long> cyc;
+  if (b->order_mode == 2 && (rc = batch_measure_cycles(b, cyc))) return rc;
   std::vector<int>& o = b->plan.order;
   o.resize(nc.size());
   for (size_t i = 0; i < o.size(); ++i) o[i] = (int)i;
@@ -148,6 +178,8 @@ static int batch_order(hcs_batch* b) {
       if (ex != ey) return tot[ex] != tot[ey] ? tot[ex] > tot[ey] : ex < ey;
       return nc[x] > nc[y];
     });
+  } else if (b->order_mode == 2) {
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return cyc[x] > cyc[y]; });
   } else {
     std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return nc[x] > nc[y]; });
   }
@@ -401,17 +433,19 @@ int hcs_batch_get_taxels(hcs_batch* b, double* out) {
 int hcs_batch_config(hcs_batch* b, int32_t* cfg) {
   if (!b || !cfg) return HCS_ERR_ARG;
   cfg[0] = b->blocks_per_cu, cfg[1] = b->qpl, cfg[2] = b->tet_max, cfg[3] = b->split_target, cfg[4] = b->queues;
+  cfg[5] = b->order_mode;
   return HCS_OK;
 }
 
 int hcs_batch_set_config(hcs_batch* b, const int32_t* cfg) {
   if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0 || cfg[3] < 0 ||
-      cfg[4] < 1 || cfg[4] > HCS_QUEUES)
+      cfg[4] < 1 || cfg[4] > HCS_QUEUES || cfg[5] < 0 || cfg[5] > 2)
     return HCS_ERR_ARG;
   /* an item holds one query per lane of the candidate kernel's wave */
   if (cfg[1] != 1)
     return set_err(b->c, HCS_ERR_ARG, "queries_per_lane must be 1 (items of more than 64 queries are switched off)");
   b->blocks_per_cu = cfg[0], b->qpl = cfg[1], b->tet_max = cfg[2], b->split_target = cfg[3], b->queues = cfg[4];
+  b->order_mode = cfg[5];
   int rc = batch_plan(b);
   if (!rc) rc = batch_capture(b);
   return rc;
@@ -482,8 +516,9 @@ int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
     if (rc) return rc;
   }
   b->split_target = bs;
-  b->order_mode = 1 - bo;
-  {
+  for (int om = 0; om < 3; ++om) {
+    if (om == bo) continue;
+    b->order_mode = om;
     int rc = try_plan(near.data(), (int)near.size());
     if (rc) return rc;
   }

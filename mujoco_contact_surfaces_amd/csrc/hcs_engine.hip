@@ -314,7 +314,7 @@ __global__ __launch_bounds__(64) void k_xtri(const EnvDesc* __restrict__ envs, c
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double y[3];
-      xform(X, E.VB + 3 * (size_t)f[i], y);
+      xform(X, hcs_plain(E.VB + 3 * (size_t)f[i]), y);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         o[3 * i + c] = y[c];
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(64) void k_xtri(const EnvDesc* __restrict__ envs, c
       }
     }
     double nS[3];
-    rot(X, E.nrmB + 3 * (size_t)t, nS);
+    rot(X, hcs_plain(E.nrmB + 3 * (size_t)t), nS);
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[9 + c] = nS[c], o[12 + c] = lo[c], o[15 + c] = hi[c];
   }
@@ -1027,11 +1027,14 @@ int build_plan(Ctx* c, int npairs, const int32_t* geoms, const hcs_step_opts* o,
       return set_err(c, HCS_ERR_ARG,
                      "pair " + std::to_string(e) + ": geom_a must be soft (pass soft-rigid pairs soft first)");
     EnvDesc& E = plan.envs[e];
-    E.recA = A.rec, E.fplA = (const float4*)A.fpl, E.boxA = A.box, E.nodesA = A.nodes, E.wnodesA = A.wnodes, E.leafA = A.leaf, E.boxLA = A.boxL, E.TA = A.E, E.VA = A.V,
+    E.recA = hcs_gptr(A.rec), E.fplA = hcs_gptr((const float4*)A.fpl), E.boxA = hcs_gptr(A.box),
+    E.nodesA = hcs_gptr(A.nodes), E.wnodesA = hcs_gptr(A.wnodes), E.leafA = hcs_gptr(A.leaf),
+    E.boxLA = hcs_gptr(A.boxL), E.TA = hcs_gptr(A.E), E.VA = hcs_gptr(A.V),
     E.ntA = A.nelem;
     E.xtri_off = -1;
-    E.VB = B.V, E.FB = B.E, E.nrmB = B.nrm, E.recB = B.rec, E.nB = B.nelem, E.nvB = B.nv;
-    E.qorder = B.kind == 2 ? B.qorder : (B.kind == 1 ? B.leaf : nullptr);
+    E.VB = hcs_gptr(B.V), E.FB = hcs_gptr(B.E), E.nrmB = hcs_gptr(B.nrm), E.recB = hcs_gptr(B.rec), E.nB = B.nelem,
+    E.nvB = B.nv;
+    E.qorder = hcs_gptr(B.kind == 2 ? B.qorder : (B.kind == 1 ? B.leaf : nullptr));
     E.b_soft = B.kind == 1;
     E.a_is_M = A.geom_id < B.geom_id;
     E.repr = o ? o->representation : 0;

@@ -98,22 +98,45 @@ enum EnvKind { KIND_TRI = 0, KIND_TET = 1, KIND_SOFT = 2, KIND_PLANE = 3 };
  * canonical candidate keys, or per-item candidate counts only (planning) */
 enum CandMode { CAND_STEP = 0, CAND_KEYS = 1, CAND_COUNT = 2 };
 
+/* Device code sees the mesh pointers of an env descriptor in the global
+ * address space: the descriptor is read from memory, so the compiler could
+ * not otherwise tell them from LDS or private pointers and would load
+ * through them with flat instructions, which count against the LDS wait
+ * counter and so make every LDS wait also wait for those loads.  Host code
+ * (and the CPU build of the device code) sees plain pointers; hcs_gptr()
+ * converts a plain pointer for assignment. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HCS_GPTR __attribute__((address_space(1)))
+#else
+#define HCS_GPTR
+#endif
+template <class T>
+__host__ __device__ inline const HCS_GPTR T* hcs_gptr(const T* p) {
+  return (const HCS_GPTR T*)p;
+}
+/* The same pointer as a plain one (for templates that deduce the element
+ * type); loads through it stay global after inlining. */
+template <class T>
+__host__ __device__ inline const T* hcs_plain(const HCS_GPTR T* p) {
+  return (const T*)p;
+}
+
 struct EnvDesc {
-  const double* recA;
-  const float4* fplA; /* A's face planes rounded to float (n, d), 4 per tet */
-  const double* boxA;
-  const Node* nodesA;
-  const WNode* wnodesA; /* the 4-wide nodes of A's LBVH, one per internal node */
-  const int32_t* leafA; /* A's tets in Morton order (tet-query items) */
-  const double* boxLA;  /* their AABBs in that order */
-  const int32_t* TA;
-  const double* VA;
-  const double* VB;
-  const int32_t* FB;  /* rigid B triangles, or soft B tets */
-  const double* nrmB; /* rigid B normals */
-  const double* recB; /* soft B records */
-  const int32_t* qorder; /* triangle / soft queries: B's elements in a
-                            spatially compact order (nullptr: identity) */
+  const HCS_GPTR double* recA;
+  const HCS_GPTR float4* fplA; /* A's face planes rounded to float (n, d), 4 per tet */
+  const HCS_GPTR double* boxA;
+  const HCS_GPTR Node* nodesA;
+  const HCS_GPTR WNode* wnodesA; /* the 4-wide nodes of A's LBVH, one per internal node */
+  const HCS_GPTR int32_t* leafA; /* A's tets in Morton order (tet-query items) */
+  const HCS_GPTR double* boxLA;  /* their AABBs in that order */
+  const HCS_GPTR int32_t* TA;
+  const HCS_GPTR double* VA;
+  const HCS_GPTR double* VB;
+  const HCS_GPTR int32_t* FB;  /* rigid B triangles, or soft B tets */
+  const HCS_GPTR double* nrmB; /* rigid B normals */
+  const HCS_GPTR double* recB; /* soft B records */
+  const HCS_GPTR int32_t* qorder; /* triangle / soft queries: B's elements in a
+                                     spatially compact order (nullptr: identity) */
   int32_t ntA, nB, kind, nvB;
   int32_t a_is_M, repr, apply, b_soft;
   double mu, dstar, dt;

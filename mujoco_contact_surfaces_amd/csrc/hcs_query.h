@@ -538,7 +538,7 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
     const int32_t* ta = E.TA + 4 * (size_t)a;
     int above = 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) above += (dot3(nS, E.VA + 3 * (size_t)ta[i]) + off) > 0;
+    for (int i = 0; i < 4; ++i) above += (dot3(nS, hcs_plain(E.VA + 3 * (size_t)ta[i])) + off) > 0;
     return above != 0 && above != 4;
   }
   if (kind != KIND_SOFT) {
@@ -810,7 +810,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
       const int32_t* f = E.FB + (size_t)nvq * e;
       for (int i = 0; i < nvq; ++i) {
         double y[3];
-        xform(X, E.VB + 3 * (size_t)f[i], y);
+        xform(X, hcs_plain(E.VB + 3 * (size_t)f[i]), y);
         if (i < 3) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) L.tv[9 * lane + 3 * i + c] = y[c];
@@ -826,7 +826,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
       }
       if (kind != KIND_SOFT) {
         double nS[3];
-        rot(X, E.nrmB + 3 * (size_t)e, nS);
+        rot(X, hcs_plain(E.nrmB + 3 * (size_t)e), nS);
 #pragma unroll
         for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
       }
@@ -878,7 +878,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                   double y[3];
-                  xform(X, E.VB + 3 * (size_t)f[i], y);
+                  xform(X, hcs_plain(E.VB + 3 * (size_t)f[i]), y);
 #pragma unroll
                   for (int c = 0; c < 3; ++c) {
                     L.tv[9 * lane + 3 * i + c] = y[c];
@@ -887,7 +887,7 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
                   }
                 }
                 double nS[3];
-                rot(X, E.nrmB + 3 * (size_t)(t0 + lane), nS);
+                rot(X, hcs_plain(E.nrmB + 3 * (size_t)(t0 + lane)), nS);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) L.tn[3 * lane + c] = nS[c];
               }

@@ -343,13 +343,14 @@ class Batch:
         return ms.value
 
     def config(self):
-        cfg = np.zeros(5, dtype=np.int32)
+        cfg = np.zeros(6, dtype=np.int32)
         self.e._chk(self.e.lib.hcs_batch_config(self.b, A.i32ptr(cfg)))
         return dict(blocks_per_cu=int(cfg[0]), queries_per_lane=int(cfg[1]), tet_query_max_tris=int(cfg[2]),
-                    split_target=int(cfg[3]), queues=int(cfg[4]))
+                    split_target=int(cfg[3]), queues=int(cfg[4]), order=int(cfg[5]))
 
-    def set_config(self, blocks_per_cu, queries_per_lane, tet_query_max_tris, split_target=2048, queues=1):
-        cfg = np.array([blocks_per_cu, queries_per_lane, tet_query_max_tris, split_target, queues], dtype=np.int32)
+    def set_config(self, blocks_per_cu, queries_per_lane, tet_query_max_tris, split_target=2048, queues=1, order=0):
+        cfg = np.array([blocks_per_cu, queries_per_lane, tet_query_max_tris, split_target, queues, order],
+                       dtype=np.int32)
         self.e._chk(self.e.lib.hcs_batch_set_config(self.b, A.i32ptr(cfg)))
 
     def step_times(self, reps=50):

@@ -247,7 +247,19 @@ def test_batched_matches_plain_and_oracle(engine):
         np.testing.assert_array_equal(cnt4, cnt)
         for i in range(len(scs)):
             assert_wrench_close(W4[i], W[i], rtol=1e-12)
+        if tet_max == 0:
+            W_tri = W4
     assert b.config()["split_target"] == 8
+    # every work order (by count, by pair, by measured cycles) gives the
+    # same candidate counts and the same slot-ordered sums
+    say("batched orders")
+    for order in (1, 2, 0):
+        b.set_config(2, 1, 0, 8, 1, order)
+        assert b.config()["order"] == order
+        b.step()
+        W5, cnt5 = b.get()
+        np.testing.assert_array_equal(cnt5, cnt)
+        np.testing.assert_array_equal(W5, W_tri)
     # items of more than 64 queries are switched off: rejected, plan kept
     from mujoco_contact_surfaces_amd.engine import HcsError
     with pytest.raises(HcsError, match="queries_per_lane"):
