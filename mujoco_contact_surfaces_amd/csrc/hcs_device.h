@@ -307,16 +307,17 @@ template <class T>
 __device__ inline bool eval_face(const PairW<T>& pc, T area, const T* cW, T p0, const T* nW, const T* gAW,
                                  const T* gBW, Acc<T>& acc, double* rec) {
   if (!(val(area) > 1.0e-14)) return false;
-  T n[3];
-  const T *gM, *gN;
-  bool M_soft, N_soft;
-  if (pc.a_is_M) {
-    n[0] = nW[0], n[1] = nW[1], n[2] = nW[2];
-    gM = gAW, gN = gBW, M_soft = true, N_soft = pc.b_soft;
-  } else {
-    n[0] = -nW[0], n[1] = -nW[1], n[2] = -nW[2];
-    gM = gBW, gN = gAW, M_soft = pc.b_soft, N_soft = true;
+  /* M / N quantities picked by value (a pointer picked between two local
+   * arrays would keep both in private memory) */
+  const bool aM = pc.a_is_M;
+  T n[3], gM[3], gN[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    n[c] = aM ? nW[c] : -nW[c];
+    gM[c] = aM ? gAW[c] : gBW[c];
+    gN[c] = aM ? gBW[c] : gAW[c];
   }
+  const bool M_soft = aM ? true : (bool)pc.b_soft, N_soft = aM ? (bool)pc.b_soft : true;
   const double kInf = __builtin_inf();
   const T gMv = M_soft ? dot3(gM, n) : T(kInf);
   const T gNv = N_soft ? -dot3(gN, n) : T(kInf);
@@ -328,10 +329,17 @@ __device__ inline bool eval_face(const PairW<T>& pc, T area, const T* cW, T p0, 
   const T g = T(1.0) / inv;
   const T fn0 = area * p0;
   const T k = area * g;
-  const T* oM = pc.a_is_M ? pc.X_WA + 9 : pc.X_WB + 9;
-  const T* oN = pc.a_is_M ? pc.X_WB + 9 : pc.X_WA + 9;
-  const T* VM = pc.a_is_M ? pc.V_WA : pc.V_WB;
-  const T* VN = pc.a_is_M ? pc.V_WB : pc.V_WA;
+  T oM[3], oN[3], VM[6], VN[6];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    oM[c] = aM ? pc.X_WA[9 + c] : pc.X_WB[9 + c];
+    oN[c] = aM ? pc.X_WB[9 + c] : pc.X_WA[9 + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    VM[c] = aM ? pc.V_WA[c] : pc.V_WB[c];
+    VN[c] = aM ? pc.V_WB[c] : pc.V_WA[c];
+  }
   /* Relative velocity at the quadrature point (plugin.cpp:440-456). */
   T rM[3] = {cW[0] - oM[0], cW[1] - oM[1], cW[2] - oM[2]};
   T rN[3] = {cW[0] - oN[0], cW[1] - oN[1], cW[2] - oN[2]};

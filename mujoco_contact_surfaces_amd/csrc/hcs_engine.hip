@@ -472,9 +472,10 @@ __global__ void k_split_roots(const EnvDesc* __restrict__ envs, const Item* __re
 
 /* Candidate generation: one wave per item; persistent when a work-queue
  * counter is given (the counter is reset by a fill kernel in the same
- * graph). */
+ * graph).  Held to two waves per SIMD: at three the triangle build's
+ * per-visit separating-axis setup spills. */
 template <int MODE, int KS>
-__global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 1) void k_cand(QueryArgs A) {
+__global__ __launch_bounds__(64, 2) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
   unsigned long long s_next = 0, s_end = 0; /* this wave's survivor slots */
   const int nq = A.live ? A.nqueues : 1;
@@ -544,9 +545,11 @@ __global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 1) void k_cand(QueryArgs A
 template <class T, int KS>
 __global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int dir_base, Contrib* out,
                                               int64_t out_stride) {
+  /* always passed by address (a null record pointer switches its output
+   * off), so the struct stays in registers */
   const OutPtrs o{A.face_rec, A.face_key, A.face_cnt, A.face_cap, A.poly_rec,
                   A.poly_key, A.poly_cnt, A.poly_cap, A.poly_tan, A.poly_env};
-  const OutPtrs* op = (A.face_rec || A.poly_rec) ? &o : nullptr;
+  const OutPtrs* op = &o;
   const int dir = dir_base < 0 ? -1 : dir_base + (int)blockIdx.y;
   Contrib* dst = out + (dir < 0 ? 0 : (int64_t)(dir - dir_base) * out_stride);
   const unsigned long long cnt = *A.surv_cnt;

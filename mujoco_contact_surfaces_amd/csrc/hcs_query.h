@@ -204,7 +204,7 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
         xform(ic.pw.X_WA, cS, cW);
         double rec[HCS_FREC];
         bool want = o && o->face_rec;
-        if (eval_face(ic.pw, area, cW, p0, nW, gAW, gBW, acc, want ? rec : nullptr)) {
+        if (eval_face(ic.pw, area, cW, p0, nW, gAW, gBW, acc, rec)) {
           ++nf;
           if (want) {
             unsigned long long slot = atomicAdd(o->face_cnt, 1ull);
@@ -224,7 +224,7 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
     xform(ic.pw.X_WA, C, cW);
     double rec[HCS_FREC];
     bool want = o && o->face_rec;
-    if (eval_face(ic.pw, area, cW, pC, nW, gAW, gBW, acc, want ? rec : nullptr)) {
+    if (eval_face(ic.pw, area, cW, pC, nW, gAW, gBW, acc, rec)) {
       ++nf;
       if (want) {
         unsigned long long slot = atomicAdd(o->face_cnt, 1ull);
@@ -435,18 +435,43 @@ __device__ inline bool box_overlap(const double* a /*lo,hi*/, const double* lo, 
  * of the projections and of the clip for meshes stored far from their
  * frame's origin or in large units. */
 #define HCS_ULP32 (32.0 * 2.220446049250313e-16)
-__device__ inline bool box_may_meet_triangle(const double* bx, const double* v, const double* n) {
-  double c[3], h[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) c[k] = 0.5 * (bx[k] + bx[3 + k]), h[k] = 0.5 * (bx[3 + k] - bx[k]);
+/* The triangle's half of the test, shared by the boxes tested against one
+ * triangle: its largest coordinate magnitude, the plane offset -n.v0, and
+ * per edge the in-plane axis m = edge x n turned away from the opposite
+ * vertex (a negation, exact) and the triangle's largest projection on it. */
+struct TriSat {
+  double m[9], tmax[3], d0, vmax;
+};
+__device__ inline void tri_sat(const double* v, const double* n, TriSat& s) {
   double vmax = 0.0;
 #pragma unroll
   for (int i = 0; i < 9; ++i) vmax = fmax(vmax, fabs(v[i]));
+  s.vmax = vmax;
+  s.d0 = -((n[0] * v[0] + n[1] * v[1]) + n[2] * v[2]);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const double* a = v + 3 * e;
+    const double* b = v + 3 * ((e + 1) % 3);
+    const double* o = v + 3 * ((e + 2) % 3);
+    const double ed[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double* m = s.m + 3 * e;
+    cross3(ed, n, m);
+    const double pa = dot3(m, a), pb = dot3(m, b), po = dot3(m, o);
+    const bool flip = po > pa;
+    s.tmax[e] = flip ? fmax(fmax(-pa, -pb), -po) : fmax(fmax(pa, pb), po);
+    if (flip) m[0] = -m[0], m[1] = -m[1], m[2] = -m[2];
+  }
+}
+/* The box's half: bx (lo, hi) against the triangle of s with unit normal n. */
+__device__ inline bool box_may_meet_tri(const double* bx, const double* n, const TriSat& s) {
+  double c[3], h[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = 0.5 * (bx[k] + bx[3 + k]), h[k] = 0.5 * (bx[3 + k] - bx[k]);
   double mag[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) mag[k] = (fabs(c[k]) + h[k]) + vmax;
+  for (int k = 0; k < 3; ++k) mag[k] = (fabs(c[k]) + h[k]) + s.vmax;
   {
-    double dist = -((n[0] * v[0] + n[1] * v[1]) + n[2] * v[2]);
+    double dist = s.d0;
     double rad = 0.0, sc = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) dist += n[k] * c[k], rad += fabs(n[k]) * h[k], sc += fabs(n[k]) * mag[k];
@@ -454,23 +479,20 @@ __device__ inline bool box_may_meet_triangle(const double* bx, const double* v, 
   }
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
-    const double* a = v + 3 * e;
-    const double* b = v + 3 * ((e + 1) % 3);
-    const double* o = v + 3 * ((e + 2) % 3);
-    const double ed[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-    double m[3];
-    cross3(ed, n, m);
-    const double pa = dot3(m, a), pb = dot3(m, b), po = dot3(m, o);
-    const double sg = po > pa ? -1.0 : 1.0;
-    const double tmax = sg > 0.0 ? fmax(fmax(pa, pb), po) : fmax(fmax(-pa, -pb), -po);
-    double bmin = sg * dot3(m, c), sc = 0.0;
+    const double* m = s.m + 3 * e;
+    double bmin = dot3(m, c), sc = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) bmin -= fabs(m[k]) * h[k], sc += fabs(m[k]) * mag[k];
     /* gap beyond the rounding margin, then beyond 1e-12 |m| (no square root) */
-    const double gap = (bmin - tmax) - HCS_ULP32 * sc;
+    const double gap = (bmin - s.tmax[e]) - HCS_ULP32 * sc;
     if (gap > 0.0 && gap * gap > 1e-24 * dot3(m, m)) return false;
   }
   return true;
+}
+__device__ inline bool box_may_meet_triangle(const double* bx, const double* v, const double* n) {
+  TriSat s;
+  tri_sat(v, n, s);
+  return box_may_meet_tri(bx, n, s);
 }
 
 
@@ -968,11 +990,6 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
               if (c == HCS_WNONE || !box_overlap(bb + 6 * k, lo, hi)) continue;
               if (c < 0) {
                 lh[k] = ~c;
-                /* a leaf whose tet box cannot meet the triangle: a
-                 * candidate the clip leaves empty, counted, not queued */
-                if (MODE == CAND_STEP && kind == KIND_TRI &&
-                    !box_may_meet_triangle(bb + 6 * k, &L.tv[9 * b], &L.tn[3 * b]))
-                  lh[k] = -1, ++nc;
                 continue;
               }
               if (sp - bp == HCS_LSTACK) {
@@ -985,6 +1002,16 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
               }
               L.stk[(sp % HCS_LSTACK) * 64 + lane] = c;
               ++sp;
+            }
+            /* leaves whose tet box cannot meet the triangle: candidates the
+             * clip leaves empty, counted, not queued (the triangle's half
+             * of the test once per visit) */
+            if (MODE == CAND_STEP && kind == KIND_TRI && (lh[0] >= 0 || lh[1] >= 0 || lh[2] >= 0 || lh[3] >= 0)) {
+              TriSat ts;
+              tri_sat(&L.tv[9 * b], &L.tn[3 * b], ts);
+#pragma unroll
+              for (int k = 0; k < 4; ++k)
+                if (lh[k] >= 0 && !box_may_meet_tri(bb + 6 * k, &L.tn[3 * b], ts)) lh[k] = -1, ++nc;
             }
           }
 #pragma unroll
