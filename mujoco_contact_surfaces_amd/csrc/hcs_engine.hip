@@ -472,10 +472,10 @@ __global__ void k_split_roots(const EnvDesc* __restrict__ envs, const Item* __re
 
 /* Candidate generation: one wave per item; persistent when a work-queue
  * counter is given (the counter is reset by a fill kernel in the same
- * graph).  Held to two waves per SIMD: at three the triangle build's
- * per-visit separating-axis setup spills. */
+ * graph).  The triangle build fits three waves per SIMD; the general build
+ * is held to two. */
 template <int MODE, int KS>
-__global__ __launch_bounds__(64, 2) void k_cand(QueryArgs A) {
+__global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 2) void k_cand(QueryArgs A) {
   __shared__ CandLds L;
   unsigned long long s_next = 0, s_end = 0; /* this wave's survivor slots */
   const int nq = A.live ? A.nqueues : 1;

@@ -462,21 +462,27 @@ __device__ inline void tri_sat(const double* v, const double* n, TriSat& s) {
     if (flip) m[0] = -m[0], m[1] = -m[1], m[2] = -m[2];
   }
 }
+/* The plane axis alone: whether box bx (lo, hi) can meet the plane
+ * n.x + d0 = 0 of a triangle with unit normal n and coordinates of
+ * magnitude at most vmax, within the margin above. */
+__device__ inline bool box_may_meet_plane(const double* bx, const double* n, double d0, double vmax) {
+  double dist = d0, rad = 0.0, sc = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double c = 0.5 * (bx[k] + bx[3 + k]), h = 0.5 * (bx[3 + k] - bx[k]);
+    dist += n[k] * c, rad += fabs(n[k]) * h, sc += fabs(n[k]) * ((fabs(c) + h) + vmax);
+  }
+  return !(fabs(dist) > rad + (1e-12 + HCS_ULP32 * sc));
+}
 /* The box's half: bx (lo, hi) against the triangle of s with unit normal n. */
 __device__ inline bool box_may_meet_tri(const double* bx, const double* n, const TriSat& s) {
+  if (!box_may_meet_plane(bx, n, s.d0, s.vmax)) return false;
   double c[3], h[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) c[k] = 0.5 * (bx[k] + bx[3 + k]), h[k] = 0.5 * (bx[3 + k] - bx[k]);
   double mag[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) mag[k] = (fabs(c[k]) + h[k]) + s.vmax;
-  {
-    double dist = s.d0;
-    double rad = 0.0, sc = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) dist += n[k] * c[k], rad += fabs(n[k]) * h[k], sc += fabs(n[k]) * mag[k];
-    if (fabs(dist) > rad + (1e-12 + HCS_ULP32 * sc)) return false;
-  }
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
     const double* m = s.m + 3 * e;
@@ -546,6 +552,53 @@ __device__ inline int64_t wave_sum_i(int64_t v) {
  *    triangle's plane or an in-plane edge axis by more than its margin.
  * soft-soft: the equal-pressure plane, both gradient filters and the
  * marching-tets sign test of process_candidate (same arithmetic). */
+/* The triangle case of survives() in two phases, so a caller can issue the
+ * loads early and test later: the tet's float face planes (64 B) and unit
+ * pressure gradient (three doubles of its record). */
+struct TriSurvIn {
+  float4 pl[4];
+  double g[3];
+};
+__device__ inline void tri_surv_load(const EnvDesc& E, int a, TriSurvIn& s) {
+  const HCS_GPTR float4* fp = E.fplA + 4 * (size_t)a;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) s.pl[k] = fp[k];
+  const HCS_GPTR double* r = E.recA + (size_t)HCS_TREC * a;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) s.g[k] = r[20 + k];
+}
+/* tv, tn: the triangle's vertices and unit normal already moved into A's
+ * frame (xform / rot with X, the same arithmetic as process_candidate).
+ * The gradient filter is the one of process_candidate (same arithmetic).
+ * The face-plane test runs on the float planes: each float differs from its
+ * double by at most 2^-24 of it, so n.v + d moves by at most
+ * 2^-24 (3 vmax + |d|) (unit n); a vertex more than twice that (plus the
+ * double test's own margin) outside is outside in the double test too, so
+ * every rejection here is one the double test makes.  The double test is
+ * not repeated for the rest: it only saves clip work, and the clip decides
+ * exactly whatever it keeps. */
+__device__ inline bool tri_surv_test(const TriSurvIn& s, const double* tv, const double* tn) {
+  double v[3][3];
+  double vmax = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    v[i][0] = tv[3 * i], v[i][1] = tv[3 * i + 1], v[i][2] = tv[3 * i + 2];
+    vmax = fmax(vmax, fmax(fabs(v[i][0]), fmax(fabs(v[i][1]), fabs(v[i][2]))));
+  }
+  const bool filt = ((s.g[0] * tn[0] + s.g[1] * tn[1]) + s.g[2] * tn[2]) > HCS_COS_ALPHA;
+  bool sep = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double n0 = s.pl[k].x, n1 = s.pl[k].y, n2 = s.pl[k].z, d = s.pl[k].w;
+    const double tol = (1.2e-7 + HCS_ULP32) * (3.0 * vmax + fabs(d)) + 1e-15;
+    bool out = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out = out && (((n0 * v[i][0] + n1 * v[i][1]) + n2 * v[i][2]) + d > tol);
+    sep = sep || out;
+  }
+  return filt && !sep;
+}
+
 template <int KS = -1>
 __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b, const double* tv, const double* tn) {
   const int kind = KS >= 0 ? KS : E.kind;
@@ -563,6 +616,11 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
     for (int i = 0; i < 4; ++i) above += (dot3(nS, hcs_plain(E.VA + 3 * (size_t)ta[i])) + off) > 0;
     return above != 0 && above != 4;
   }
+  if (kind != KIND_SOFT && E.fplA) {
+    TriSurvIn s;
+    tri_surv_load(E, a, s);
+    return tri_surv_test(s, tv, tn);
+  }
   if (kind != KIND_SOFT) {
     /* tv, tn: triangle b's vertices and unit normal already moved into A's
      * frame (xform / rot with X, the same arithmetic as process_candidate) */
@@ -573,33 +631,10 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
       v[i][0] = tv[3 * i], v[i][1] = tv[3 * i + 1], v[i][2] = tv[3 * i + 2];
       vmax = fmax(vmax, fmax(fabs(v[i][0]), fmax(fabs(v[i][1]), fabs(v[i][2]))));
     }
-    /* the gradient filter, computed before the plane test so that both
-     * loads are in flight together */
     const double nS[3] = {tn[0], tn[1], tn[2]};
     const bool filt = ((r[20] * nS[0] + r[21] * nS[1]) + r[22] * nS[2]) > HCS_COS_ALPHA;
     bool sep = false;
-    if (E.fplA) {
-      /* the face-plane test on the float planes (64 B instead of 128 B of
-       * the record): each float differs from its double by at most 2^-24
-       * of it, so n.v + d moves by at most 2^-24 (3 vmax + |d|) (unit n);
-       * a vertex more than twice that (plus the double test's own margin)
-       * outside is outside in the double test too, so every rejection here is one the double
-       * test makes.  The double test is not repeated for the rest: it only
-       * saves clip work, and the clip decides exactly whatever it keeps. */
-      const float4* fp = E.fplA + 4 * (size_t)a;
-      float4 pl[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) pl[k] = fp[k];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const double n0 = pl[k].x, n1 = pl[k].y, n2 = pl[k].z, d = pl[k].w;
-        const double tol = (1.2e-7 + HCS_ULP32) * (3.0 * vmax + fabs(d)) + 1e-15;
-        bool out = true;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) out = out && (((n0 * v[i][0] + n1 * v[i][1]) + n2 * v[i][2]) + d > tol);
-        sep = sep || out;
-      }
-    } else {
+    { /* no float planes: the face-plane test on the record's double planes */
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         const double* n = r + 3 * k;
@@ -1003,15 +1038,21 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
               L.stk[(sp % HCS_LSTACK) * 64 + lane] = c;
               ++sp;
             }
-            /* leaves whose tet box cannot meet the triangle: candidates the
-             * clip leaves empty, counted, not queued (the triangle's half
-             * of the test once per visit) */
+            /* leaves whose tet box the triangle's plane separates from the
+             * triangle: candidates the clip leaves empty, counted, not
+             * queued.  Only the plane axis is tested here (the in-plane edge
+             * axes of box_may_meet_triangle cost more at this point than
+             * the few more rejections they make save later). */
             if (MODE == CAND_STEP && kind == KIND_TRI && (lh[0] >= 0 || lh[1] >= 0 || lh[2] >= 0 || lh[3] >= 0)) {
-              TriSat ts;
-              tri_sat(&L.tv[9 * b], &L.tn[3 * b], ts);
+              const double* v = &L.tv[9 * b];
+              const double* n = &L.tn[3 * b];
+              double vmax = 0.0;
+#pragma unroll
+              for (int i = 0; i < 9; ++i) vmax = fmax(vmax, fabs(v[i]));
+              const double d0 = -((n[0] * v[0] + n[1] * v[1]) + n[2] * v[2]);
 #pragma unroll
               for (int k = 0; k < 4; ++k)
-                if (lh[k] >= 0 && !box_may_meet_tri(bb + 6 * k, &L.tn[3 * b], ts)) lh[k] = -1, ++nc;
+                if (lh[k] >= 0 && !box_may_meet_plane(bb + 6 * k, n, d0, vmax)) lh[k] = -1, ++nc;
             }
           }
 #pragma unroll
