@@ -334,9 +334,12 @@ __global__ __launch_bounds__(64) void k_xtri(const EnvDesc* __restrict__ envs, c
  * [6..11] the target (B's vertex AABB in B's frame for tet queries, the
  * item's subtree of A's LBVH otherwise). */
 __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
-                                                   int nitems, double* __restrict__ ibox) {
-  const int i = blockIdx.x, lane = threadIdx.x;
-  if (i >= nitems) return;
+                                                   const int* __restrict__ order, int nitems,
+                                                   double* __restrict__ ibox) {
+  /* block k: the k-th item of the work order, its boxes at position k */
+  const int k = blockIdx.x, lane = threadIdx.x;
+  if (k >= nitems) return;
+  const int i = order ? order[k] : k;
   const Item it = items[i];
   const EnvDesc& E = envs[it.env];
   const double inf = __builtin_inf();
@@ -382,7 +385,7 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
     t[c] = wave_min(t[c]), t[3 + c] = wave_max(t[3 + c]);
   }
   if (lane == 0) {
-    double* o = ibox + 12 * (size_t)i;
+    double* o = ibox + 12 * (size_t)k;
 #pragma unroll
     for (int c = 0; c < 3; ++c) o[c] = lo[c], o[3 + c] = hi[c], o[6 + c] = t[c], o[9 + c] = t[3 + c];
   }
@@ -392,8 +395,10 @@ __global__ __launch_bounds__(64) void k_item_boxes(const EnvDesc* __restrict__ e
  * overlap under this step's pose, appended to their queue's segment of the
  * live list (one atomic per queue present in the wave); a rejected item gets
  * zero counts here. */
-__global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
-                                              const int* __restrict__ order, int nitems,
+#define HCS_CULL_THREADS 1024
+__global__ __launch_bounds__(HCS_CULL_THREADS) void k_cull(const EnvDesc* __restrict__ envs, const Item* __restrict__ items,
+                                              const int* __restrict__ order, const Item* __restrict__ items_o,
+                                              int nitems,
                                               const double* __restrict__ ibox,
                                               const double* __restrict__ xf, const int* __restrict__ qseg,
                                               int nq, int* __restrict__ live, int4* __restrict__ live_item,
@@ -402,14 +407,15 @@ __global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, 
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   /* threads visit the items in the plan's work order (heaviest first), so
-   * the live queues hand out long items before short ones */
+   * the live queues hand out long items before short ones; the item records
+   * and boxes are stored in that order, so the reads are contiguous */
   const int i = (k < nitems && order) ? order[k] : k;
   bool keep = false;
   int q = -1;
   Item it{0, 0, 0, 0};
   if (k < nitems) {
-    it = items[i];
-    const double* bx = ibox + 12 * (size_t)i;
+    it = order ? items_o[k] : items[k];
+    const double* bx = ibox + 12 * (size_t)k;
     const int kind = envs[it.env].kind;
     const double* X = xf + 12 * (size_t)it.env;
     if (kind == KIND_PLANE) {
@@ -420,6 +426,31 @@ __global__ __launch_bounds__(256) void k_cull(const EnvDesc* __restrict__ envs, 
     }
     if (!keep) ncand[i] = 0, nsurv[i] = 0;
     q = it.env % nq;
+  }
+  if (nq == 1) {
+    /* one queue: one atomic per block on its counter (one per wave
+     * contends on that single address), the block's waves in order */
+    __shared__ unsigned s_off[HCS_CULL_THREADS / 64 + 1];
+    const int w = threadIdx.x >> 6;
+    const unsigned long long m = __ballot(keep);
+    if (lane == 0) s_off[w] = (unsigned)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned tot = 0;
+      for (int x = 0; x < (int)(blockDim.x >> 6); ++x) {
+        const unsigned n = s_off[x];
+        s_off[x] = tot;
+        tot += n;
+      }
+      s_off[HCS_CULL_THREADS / 64] = atomicAdd(live_cnt, tot);
+    }
+    __syncthreads();
+    if (keep) {
+      const int pos = qseg[0] + s_off[HCS_CULL_THREADS / 64] + s_off[w] + lane_prefix(m);
+      live[pos] = i;
+      live_item[pos] = make_int4(it.env, it.first, it.count, it.root);
+    }
+    return;
   }
   unsigned long long rem = __ballot(keep);
   while (rem) {
@@ -696,8 +727,9 @@ void pipe_front(Ctx* c, Pipe& p, const QueryArgs& a, int npairs) {
     hipLaunchKernelGGL(k_xtri, dim3(p.nxenv), dim3(64), 0, s, a.envs, (const int*)p.xenv.ptr,
                        (const double*)p.xf.ptr, (double*)p.xtri.ptr);
   if (a.nitems > 0 && a.live)
-    hipLaunchKernelGGL(k_cull, dim3((a.nitems + 255) / 256), dim3(256), 0, s, a.envs, a.items,
-                       p.ordered ? (const int*)p.order.ptr : nullptr, a.nitems,
+    hipLaunchKernelGGL(k_cull, dim3((a.nitems + HCS_CULL_THREADS - 1) / HCS_CULL_THREADS), dim3(HCS_CULL_THREADS), 0,
+                       s, a.envs, a.items,
+                       p.ordered ? (const int*)p.order.ptr : nullptr, (const Item*)p.item_o.ptr, a.nitems,
                        (const double*)p.ibox.ptr, (const double*)p.xf.ptr, (const int*)p.qseg.ptr, p.nqueues,
                        (int*)p.live.ptr, (int4*)p.live_item.ptr, (unsigned*)p.counter.ptr + HCS_QUEUES,
                        a.item_ncand, a.item_nsurv);
@@ -830,9 +862,14 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
   }
   p.ordered = (int)plan.order.size() == nitems && nitems > 0;
   if (p.ordered) {
-    if ((rc = ensure(c, p.order, sizeof(int) * ni))) return rc;
+    if ((rc = ensure(c, p.order, sizeof(int) * ni)) || (rc = ensure(c, p.item_o, sizeof(Item) * ni))) return rc;
+    p.item_o_host.resize(nitems);
+    for (int k = 0; k < nitems; ++k) p.item_o_host[k] = host_items[plan.order[k]];
     rc = hip_check(c, hipMemcpyAsync(p.order.ptr, plan.order.data(), sizeof(int) * nitems, hipMemcpyHostToDevice,
                                      c->stream), "upload order");
+    if (!rc)
+      rc = hip_check(c, hipMemcpyAsync(p.item_o.ptr, p.item_o_host.data(), sizeof(Item) * nitems,
+                                       hipMemcpyHostToDevice, c->stream), "upload ordered items");
     if (rc) return rc;
   }
   rc = hip_check(c, hipMemcpyAsync(p.qseg.ptr, p.qseg_host.data(), sizeof(int) * (HCS_QUEUES + 1),
@@ -841,8 +878,9 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
     rc = hip_check(c, hipMemcpyAsync(p.xenv.ptr, plan.xenv.data(), sizeof(int) * p.nxenv, hipMemcpyHostToDevice,
                                      c->stream), "upload tet-query pairs");
   if (rc) return rc;
-  if (nitems > 0) hipLaunchKernelGGL(k_item_boxes, dim3(nitems), dim3(64), 0, c->stream, envs, items, nitems,
-                                     (double*)p.ibox.ptr);
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_item_boxes, dim3(nitems), dim3(64), 0, c->stream, envs, items,
+                       p.ordered ? (const int*)p.order.ptr : nullptr, nitems, (double*)p.ibox.ptr);
   rc = hip_check(c, hipGetLastError(), "item boxes");
   if (!rc) rc = hip_check(c, hipStreamSynchronize(c->stream), "sync");
   return rc;
@@ -850,7 +888,7 @@ int pipe_plan(Ctx* c, Pipe& p, const EnvDesc* envs, const Item* items, const Pla
 
 void pipe_free(Pipe& p) {
   for (DBuf* b : {&p.counter, &p.surv, &p.surv_cnt, &p.ncand, &p.nsurv, &p.off, &p.scan_tmp, &p.contrib, &p.out, &p.err,
-                  &p.xf, &p.ibox, &p.live, &p.live_item, &p.qseg, &p.xtri, &p.xenv, &p.order, &p.part, &p.rtab,
+                  &p.xf, &p.ibox, &p.live, &p.live_item, &p.qseg, &p.xtri, &p.xenv, &p.order, &p.item_o, &p.part, &p.rtab,
                   &p.ptab})
     if (b->ptr) (void)hipFree(b->ptr), b->ptr = nullptr, b->bytes = 0;
 }

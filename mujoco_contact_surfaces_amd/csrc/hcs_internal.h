@@ -46,11 +46,13 @@ struct DBuf {
  * per-item survivor counts, k_clip, k_env_reduce. */
 struct Pipe {
   DBuf counter /* [0..7] work counters, [8..15] live items per queue */, surv, surv_cnt, ncand, nsurv, off, scan_tmp,
-      contrib, out, err, xf, ibox, live, live_item, qseg, xtri, xenv, order, part /* per-slice sums */,
+      contrib, out, err, xf, ibox /* item boxes, in the work order when ordered */, live, live_item, qseg, xtri, xenv,
+      order, item_o /* the items in the work order (ordered plans) */, part /* per-slice sums */,
       rtab /* reduction waves (pair, slice, slices, first partial) */, ptab /* per pair (first partial, slices) */;
   int64_t cap = 0;
   size_t scan_bytes = 0;
   std::vector<int> qseg_host; /* host copy of the queue segments (the upload source) */
+  std::vector<Item> item_o_host; /* upload source of item_o */
   int nqueues = 1;            /* candidate-kernel work queues (1..HCS_QUEUES) */
   int nxenv = 0;              /* pairs in tet-query mode (their triangles are moved per step) */
   bool ordered = false;       /* order holds a work order of the items */
