@@ -25,7 +25,12 @@ prints, for the last dispatch of every kernel matching --kernels:
     (Num_Xcc of the run's agent_info.csv) to give the dispatch's clocks;
   * vector-memory path: TD_TD_BUSY_sum and TA_TA_BUSY_sum over GRBM_GUI_ACTIVE
     and CUs (busy fraction of the texture data / address units), the mean L2
-    read latency TCP_TCC_READ_REQ_LATENCY_sum / TCP_TCC_READ_REQ_sum (cycles).
+    read latency TCP_TCC_READ_REQ_LATENCY_sum / TCP_TCC_READ_REQ_sum (cycles);
+  * vector ALU: SQ_INSTS_VALU over GRBM_GUI_ACTIVE and CUs (issue use: a
+    CU's four SIMDs issue one wave64 instruction each per four cycles, so one
+    per cycle per CU at most), SQ_THREAD_CYCLES_VALU / (64 SQ_INSTS_VALU)
+    (active lanes per instruction), SQ_WAIT_ANY / SQ_WAVE_CYCLES (share of
+    the waves' time spent waiting).
 Counters a run did not collect are left out.  Nothing here runs on the GPU.
 """
 import argparse
@@ -158,7 +163,9 @@ def counters(args):
             args.calib_mib, ", ".join("%.3f (%s)" % (v, k) for k, v in scale.items())))
         lines.append("")
     cols = ["FETCH_SIZE", "WRITE_SIZE", "SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE",
-            "TD_TD_BUSY_sum", "TA_TA_BUSY_sum", "TCP_TCC_READ_REQ_LATENCY_sum", "TCP_TCC_READ_REQ_sum"]
+            "TD_TD_BUSY_sum", "TA_TA_BUSY_sum", "TCP_TCC_READ_REQ_LATENCY_sum", "TCP_TCC_READ_REQ_sum",
+            "SQ_INSTS_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAIT_ANY"]
+    cols = [c for c in cols if any(c in v for v in table.values())]
     lines.append("Last dispatch of each kernel:")
     lines.append("")
     lines.append("| kernel | " + " | ".join(c for c in cols) + " |")
@@ -185,6 +192,12 @@ def counters(args):
                 dv["td_busy_pct"] = 100.0 * v["TD_TD_BUSY_sum"] / g / args.cus
             if "TA_TA_BUSY_sum" in v:
                 dv["ta_busy_pct"] = 100.0 * v["TA_TA_BUSY_sum"] / g / args.cus
+            if "SQ_INSTS_VALU" in v:
+                dv["valu_issue_pct"] = 100.0 * v["SQ_INSTS_VALU"] / g / args.cus
+        if v.get("SQ_INSTS_VALU") and "SQ_THREAD_CYCLES_VALU" in v:
+            dv["valu_lanes_pct"] = 100.0 * v["SQ_THREAD_CYCLES_VALU"] / (64.0 * v["SQ_INSTS_VALU"])
+        if v.get("SQ_WAVE_CYCLES") and "SQ_WAIT_ANY" in v:
+            dv["wait_pct"] = 100.0 * v["SQ_WAIT_ANY"] / v["SQ_WAVE_CYCLES"]
         if v.get("TCP_TCC_READ_REQ_sum"):
             dv["l2_read_latency_cycles"] = v.get("TCP_TCC_READ_REQ_LATENCY_sum", 0.0) / v["TCP_TCC_READ_REQ_sum"]
         if "SQ_WAVES" in v:
@@ -193,18 +206,20 @@ def counters(args):
     lines += ["", "Derived (GRBM_GUI_ACTIVE / %d XCDs is the dispatch's duration in GPU clocks; %d CUs):" % (xcc, args.cus),
               "",
               "| kernel | HBM read GB | HBM write GB | HBM GB | mean waves / CU | occupancy % of 32 | TD busy % | "
-              "TA busy % | L2 read latency (cycles) |", "|---|---|---|---|---|---|---|---|---|"]
+              "TA busy % | L2 read latency (cycles) | VALU issue % | VALU lanes % | waiting % |",
+              "|---|---|---|---|---|---|---|---|---|---|---|---|"]
 
     def f(x, fmt):
         return fmt % x if x is not None else ""
 
     for k, dv in derived.items():
-        lines.append("| `%s` | %s | %s | %s | %s | %s | %s | %s | %s |" % (
+        lines.append("| `%s` | %s | %s | %s | %s | %s | %s | %s | %s | %s | %s | %s |" % (
             _short(k, 50), f(dv.get("hbm_read_bytes", None) and dv["hbm_read_bytes"] / 1e9, "%.3f"),
             f(dv.get("hbm_write_bytes", None) and dv["hbm_write_bytes"] / 1e9, "%.3f"),
             f(dv.get("hbm_bytes", None) and dv["hbm_bytes"] / 1e9, "%.3f"), f(dv.get("mean_waves_per_cu"), "%.2f"),
             f(dv.get("occupancy_pct_of_32"), "%.1f"), f(dv.get("td_busy_pct"), "%.1f"),
-            f(dv.get("ta_busy_pct"), "%.1f"), f(dv.get("l2_read_latency_cycles"), "%.0f")))
+            f(dv.get("ta_busy_pct"), "%.1f"), f(dv.get("l2_read_latency_cycles"), "%.0f"),
+            f(dv.get("valu_issue_pct"), "%.1f"), f(dv.get("valu_lanes_pct"), "%.1f"), f(dv.get("wait_pct"), "%.1f")))
     print("\n".join(lines))
     if args.json:
         short = {}

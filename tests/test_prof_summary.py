@@ -62,3 +62,25 @@ def test_counter_scaling_and_derived_figures(tmp_path, capsys):
     assert abs(d["td_busy_pct"] - 50.0) < 1e-12 and abs(d["ta_busy_pct"] - 25.0) < 1e-12
     assert d["l2_read_latency_cycles"] == 800.0
     assert json.load(open(js))["hbm_bytes_per_launch"]["k_cand"] == 1000.0 * 1024 * 2.0 + 10.0 * 1024
+
+
+def test_xcd_count_and_valu_figures(tmp_path, capsys):
+    """GRBM_GUI_ACTIVE is summed over the XCDs that agent_info.csv names;
+    VALU issue use, active lanes and waiting share."""
+    k = "void hcs::k_clip<double, 0>(hcs::QueryArgs)"
+    p = str(tmp_path / "p")
+    _counters(p, k, {"GRBM_GUI_ACTIVE": 8 * 1000.0, "SQ_INSTS_VALU": 0.5 * 1000 * 256,
+                     "SQ_THREAD_CYCLES_VALU": 0.25 * 64 * 0.5 * 1000 * 256, "SQ_WAVE_CYCLES": 400.0,
+                     "SQ_WAIT_ANY": 100.0, "TD_TD_BUSY_sum": 0.5 * 256 * 1000})
+    _write(os.path.join(p, "host", "123_agent_info.csv"), ["Node_Id", "Agent_Type", "Num_Xcc"],
+           [[0, "CPU", 0], [1, "GPU", 8]])
+    js = str(tmp_path / "out.json")
+    P.main(["counters", "--kernels", "k_clip", "--pass", p, "--json", js])
+    out = capsys.readouterr().out
+    assert "/ 8 XCDs" in out
+    d = json.load(open(js))["per_kernel"]["k_clip"]
+    assert d["gpu_clocks"] == 1000.0
+    assert abs(d["td_busy_pct"] - 50.0) < 1e-12
+    assert abs(d["valu_issue_pct"] - 50.0) < 1e-12
+    assert abs(d["valu_lanes_pct"] - 25.0) < 1e-12
+    assert abs(d["wait_pct"] - 25.0) < 1e-12
