@@ -234,6 +234,127 @@ __device__ inline void clip_quad_tet(Poly<T>& p, const T* nrm, const T* off) {
   dedupe<T, 8>(p);
 }
 
+/* ------------------------------------------- polygons in a memory column */
+
+/* A polygon of at most HCS_MAXV vertices kept in memory, one column per
+ * thread (LDS in the clip kernel): coordinate c of vertex k at
+ * col[(3 k + c) * st].  The clip below is clip_halfspace / dedupe with the
+ * same arithmetic on the same values, but a vertex goes to its output
+ * position with one store rather than a select over every position. */
+struct ColPoly {
+  double* col;
+  int st, n;
+  __device__ double& at(int k, int c) const { return col[(3 * k + c) * st]; }
+};
+
+template <int NIN>
+__device__ inline void clip_halfspace_col(ColPoly& P, const double* nh, double off) {
+  const int n = P.n;
+  /* the whole input first: the outputs overwrite it */
+  double x[NIN], y[NIN], z[NIN], s[NIN];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) {
+    x[i] = P.at(i, 0), y[i] = P.at(i, 1), z[i] = P.at(i, 2);
+    s[i] = ((nh[0] * x[i] + nh[1] * y[i]) + nh[2] * z[i]) + off;
+  }
+  const double sl = pick<double, NIN>(s, n - 1);
+  const double xl = pick<double, NIN>(x, n - 1), yl = pick<double, NIN>(y, n - 1), zl = pick<double, NIN>(z, n - 1);
+  int m = 0;
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) {
+    const bool valid = i < n;
+    const double sp = (i == 0) ? sl : s[i - 1];
+    const double px = (i == 0) ? xl : x[i - 1];
+    const double py = (i == 0) ? yl : y[i - 1];
+    const double pz = (i == 0) ? zl : z[i - 1];
+    const bool cin = s[i] <= 0, pin = sp <= 0;
+    if (valid && (cin != pin) && m < NIN + 1) {
+      const double wa = sp / (sp - s[i]);
+      const double wb = 1.0 - wa;
+      P.at(m, 0) = wa * x[i] + wb * px;
+      P.at(m, 1) = wa * y[i] + wb * py;
+      P.at(m, 2) = wa * z[i] + wb * pz;
+      ++m;
+    }
+    if (valid && cin && m < NIN + 1) {
+      P.at(m, 0) = x[i], P.at(m, 1) = y[i], P.at(m, 2) = z[i];
+      ++m;
+    }
+  }
+  P.n = m;
+}
+
+template <int NMAX>
+__device__ inline void dedupe_col(ColPoly& P) {
+  const int n = P.n;
+  if (n <= 0) {
+    P.n = 0;
+    return;
+  }
+  double x[NMAX], y[NMAX], z[NMAX];
+#pragma unroll
+  for (int i = 0; i < NMAX; ++i) x[i] = P.at(i, 0), y[i] = P.at(i, 1), z[i] = P.at(i, 2);
+  double lx = x[0], ly = y[0], lz = z[0];
+  int m = 1;
+#pragma unroll
+  for (int i = 1; i < NMAX; ++i) {
+    if (i < n) {
+      const double dx = lx - x[i], dy = ly - y[i], dz = lz - z[i];
+      if (!(((dx * dx + dy * dy) + dz * dz) < HCS_EPS2)) {
+        P.at(m, 0) = x[i], P.at(m, 1) = y[i], P.at(m, 2) = z[i];
+        lx = x[i], ly = y[i], lz = z[i];
+        ++m;
+      }
+    }
+  }
+  if (m >= 3) {
+    const double dx = lx - x[0], dy = ly - y[0], dz = lz - z[0];
+    if (((dx * dx + dy * dy) + dz * dz) < HCS_EPS2) --m;
+  }
+  P.n = m >= 3 ? m : 0;
+}
+
+/* clip_tri_tet on a column polygon holding the triangle */
+template <class R>
+__device__ inline void clip_tri_tet_col(ColPoly& P, const R* rec) {
+  double nh[3] = {rec[0], rec[1], rec[2]};
+  clip_halfspace_col<3>(P, nh, rec[12]);
+  if (P.n == 0) return;
+  nh[0] = rec[3], nh[1] = rec[4], nh[2] = rec[5];
+  clip_halfspace_col<4>(P, nh, rec[13]);
+  if (P.n == 0) return;
+  nh[0] = rec[6], nh[1] = rec[7], nh[2] = rec[8];
+  clip_halfspace_col<5>(P, nh, rec[14]);
+  if (P.n == 0) return;
+  nh[0] = rec[9], nh[1] = rec[10], nh[2] = rec[11];
+  clip_halfspace_col<6>(P, nh, rec[15]);
+  dedupe_col<7>(P);
+}
+
+/* clip_quad_tet on a column polygon */
+__device__ inline void clip_quad_tet_col(ColPoly& P, const double* nrm, const double* off) {
+  clip_halfspace_col<4>(P, nrm, off[0]);
+  if (P.n == 0) return;
+  clip_halfspace_col<5>(P, nrm + 3, off[1]);
+  if (P.n == 0) return;
+  clip_halfspace_col<6>(P, nrm + 6, off[2]);
+  if (P.n == 0) return;
+  clip_halfspace_col<7>(P, nrm + 9, off[3]);
+  dedupe_col<8>(P);
+}
+
+/* The first NMAX vertices of a column polygon into registers (positions
+ * at and past n hold whatever the column holds; no reader uses them). */
+template <int NMAX>
+__device__ inline void col_to_poly(const ColPoly& P, Poly<double>& p) {
+#pragma unroll
+  for (int k = 0; k < HCS_MAXV; ++k) {
+    if (k < NMAX) p.x[k] = P.at(k, 0), p.y[k] = P.at(k, 1), p.z[k] = P.at(k, 2);
+    else p.x[k] = p.y[k] = p.z[k] = 0.0;
+  }
+  p.n = P.n;
+}
+
 /* Drake semantics (CalcPolygonCentroid): area-weighted centroid of the fan
  * from v0 with weights (v_i - v0) x (v_{i+1} - v0) . n; the vertex mean when
  * the weights sum to zero.  Returns the weight sum (twice the area). */

@@ -22,6 +22,7 @@
 #include <limits>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hcs_internal.h"
@@ -585,8 +586,12 @@ __global__ __launch_bounds__(256, 2) void k_clip(QueryArgs A, ClipArgs C, int di
   Contrib* dst = out + (dir < 0 ? 0 : (int64_t)(dir - dir_base) * out_stride);
   const unsigned long long cnt = *A.surv_cnt;
   const int64_t n = (int64_t)cnt < A.surv_cap ? (int64_t)cnt : A.surv_cap;
+  /* the double build clips in LDS, one column of HCS_MAXV vertices per
+   * thread (48 KB per block) */
+  constexpr bool kCol = std::is_same<T, double>::value;
+  __shared__ double col[kCol ? 3 * HCS_MAXV * 256 : 1];
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    clip_survivor<T, KS>(A, C, i, dir, dst, op);
+    clip_survivor<T, KS, kCol>(A, C, i, dir, dst, op, col + threadIdx.x, 256);
 }
 
 /* Per pair, in K_e waves: wave w = (pair e, slice j, K_e, first partial)

@@ -337,10 +337,13 @@ __device__ inline bool slice_tet(const EnvDesc& E, int a, const T* nh, T off, Po
 }
 
 /* KS: the pair kind when every pair of the launch has it (the kind branches
- * fold away), or -1 for a launch that mixes kinds. */
-template <class T, int KS = -1>
+ * fold away), or -1 for a launch that mixes kinds.  COL (double only): the
+ * clip runs on a column polygon at colb with stride cols (ColPoly) instead
+ * of in registers; the results are the same. */
+template <class T, int KS = -1, bool COL = false>
 __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic, int env, int a, int b, Acc<T>& acc,
-                                         int64_t& np, int64_t& nf, const OutPtrs* o) {
+                                         int64_t& np, int64_t& nf, const OutPtrs* o, double* colb = nullptr,
+                                         int cols = 1) {
   const int kind = KS >= 0 ? KS : E.kind;
   const double* __restrict__ rr = E.recA + (size_t)HCS_TREC * a;
   T gA[3] = {ic.escale * rr[16], ic.escale * rr[17], ic.escale * rr[18]};
@@ -377,7 +380,16 @@ __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic,
     rot(ic.X_AB, nR, nS);
     /* Drake semantics (IsFaceNormalAlongPressureGradient) */
     if (!(((rr[20] * val(nS[0]) + rr[21] * val(nS[1])) + rr[22] * val(nS[2])) > HCS_COS_ALPHA)) return;
-    clip_tri_tet(p, rr);
+    if constexpr (COL) {
+      ColPoly P{colb, cols, 3};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) P.at(i, 0) = p.x[i], P.at(i, 1) = p.y[i], P.at(i, 2) = p.z[i];
+      clip_tri_tet_col(P, rr);
+      if (P.n < 3) return;
+      col_to_poly<7>(P, p);
+    } else {
+      clip_tri_tet(p, rr);
+    }
     if (p.n < 3) return;
     ++np;
     T zero3[3] = {T(0.0), T(0.0), T(0.0)};
@@ -409,7 +421,16 @@ __device__ inline void process_candidate(const EnvDesc& E, const ItemCtx<T>& ic,
       rot(ic.X_AB, nb, pn + 3 * k);
       po[k] = T(rbb[12 + k]) - dot3(pn + 3 * k, ic.X_AB + 9);
     }
-    clip_quad_tet(p, pn, po);
+    if constexpr (COL) {
+      ColPoly P{colb, cols, p.n};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) P.at(i, 0) = p.x[i], P.at(i, 1) = p.y[i], P.at(i, 2) = p.z[i];
+      clip_quad_tet_col(P, pn, po);
+      if (P.n < 3) return;
+      col_to_poly<8>(P, p);
+    } else {
+      clip_quad_tet(p, pn, po);
+    }
     if (p.n < 3) return;
     ++np;
     T gBW[3];
@@ -1136,9 +1157,9 @@ __device__ inline void run_cand(CandLds& L, const QueryArgs& A, int it, const It
  * k_clip: one survivor per thread -> contribution at item_off + seq
  * =================================================================== */
 
-template <class T, int KS = -1>
+template <class T, int KS = -1, bool COL = false>
 __device__ inline void clip_survivor(const QueryArgs& A, const ClipArgs& C, int64_t i, int dir, Contrib* out,
-                                     const OutPtrs* o) {
+                                     const OutPtrs* o, double* colb = nullptr, int cols = 1) {
   const int4 s = A.surv[i];
   if (s.x < 0) return; /* unused slot of a wave's reserved chunk */
   const int64_t slot = (int64_t)C.item_off[s.z] + (unsigned)s.w;
@@ -1151,7 +1172,7 @@ __device__ inline void clip_survivor(const QueryArgs& A, const ClipArgs& C, int6
   Acc<T> acc;
   acc.zero();
   int64_t np = 0, nf = 0;
-  process_candidate<T, KS>(E, ic, env, s.x, s.y, acc, np, nf, o);
+  process_candidate<T, KS, COL>(E, ic, env, s.x, s.y, acc, np, nf, o, colb, cols);
   Contrib r;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {

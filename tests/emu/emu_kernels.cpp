@@ -277,10 +277,14 @@ int emu_pair(const double* VA, int nva, const int32_t* TA, int nta, const double
   std::vector<Contrib> con(std::max<unsigned>(1, off[nitems]));
   for (int dir = -1; dir < ndir; ++dir) {
     for (auto& q : con) q = Contrib{};
+    /* the double clips run on a column polygon (as in the clip kernel), the
+     * general build and the tangent pass on register polygons */
+    double colbuf[3 * HCS_MAXV] = {};
     for (unsigned long long i = 0; i < scnt; ++i) {
-      if (dir < 0 && E.kind == KIND_TRI && !g_generic) clip_survivor<double, KIND_TRI>(Q, C, (int64_t)i, dir, con.data(), nullptr);
+      if (dir < 0 && E.kind == KIND_TRI && !g_generic)
+        clip_survivor<double, KIND_TRI, true>(Q, C, (int64_t)i, dir, con.data(), nullptr, colbuf, 1);
       else if (dir < 0 && E.kind == KIND_SOFT && !g_generic)
-        clip_survivor<double, KIND_SOFT>(Q, C, (int64_t)i, dir, con.data(), nullptr);
+        clip_survivor<double, KIND_SOFT, true>(Q, C, (int64_t)i, dir, con.data(), nullptr, colbuf, 1);
       else if (dir < 0) clip_survivor<double>(Q, C, (int64_t)i, dir, con.data(), nullptr);
       else clip_survivor<Dual>(Q, C, (int64_t)i, dir, con.data(), nullptr);
     }
