@@ -276,15 +276,24 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
 
 /* ---- fan-major sampling (double, S^2 <= HCS_TX_RAYS): a lane takes a fan
  * triangle and runs the hit test on the rays of its sample range only; the
- * nearest hit per ray is found with LDS atomics in two passes: the least t
- * (its IEEE bits order like t, t > 0), then among hits at that t the least
- * (polygon key, fan index), the tie rule of the ray-major loop above.  The
- * packed key kq * 8 + k needs kq < 2^61, i.e. pair indices below 2^19.
- * Every hit test is the ray-major loop's arithmetic on the same values, so
- * the chosen triangle, and the taxel, are the same. */
+ * nearest hit per ray is found with LDS atomics: the least t (its IEEE bits
+ * order like t, t > 0), then among hits at that t the least (polygon key,
+ * fan index), the tie rule of the ray-major loop above.  The packed key
+ * kq * 8 + k needs kq < 2^61, i.e. pair indices below 2^19.
+ *
+ * The first pass also records, per ray, the fan whose atomic lowered t
+ * (the owner) and flags a ray when a hit matched the t already there (a
+ * tie).  A ray whose owner's recomputed t is its least t and that has no
+ * tie is settled; only the other rays (and every ray of a taxel whose
+ * polygon list takes more than one chunk) run the second pass, which finds
+ * the least key among the hits at the least t.  Every hit test is the
+ * ray-major loop's arithmetic on the same values, so the chosen triangle,
+ * and the taxel, are the same. */
 #define HCS_TX_RAYS 576
 struct TxScatter {
   unsigned long long tmin[HCS_TX_RAYS], smin[HCS_TX_RAYS];
+  int flag[HCS_TX_RAYS]; /* the ray needs the second pass */
+  int slot[HCS_TX_RAYS]; /* a settled ray: the list slot of its polygon (else -1) */
   int q[HCS_TX_CHUNK], fan0[HCS_TX_CHUNK + 1];
   long long kq[HCS_TX_CHUNK];
 };
@@ -377,12 +386,18 @@ __device__ inline void scatter_chunk(TxScatter& L, const double* __restrict__ po
     const unsigned long long sk = ((unsigned long long)L.kq[l] << 3) | (unsigned)k;
     for (int i = c.i0; i <= c.i1; ++i)
       for (int j = c.j0; j <= c.j1; ++j) {
+        const int s = i * g.S + j;
+        if (PASS == 1 && !L.flag[s]) continue;
         const double t = fan_hit(c, g, x, y, i, j);
         if (!(t > 0.0)) continue;
-        const int s = i * g.S + j;
         const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
-        if (PASS == 0) atomicMin(&L.tmin[s], tb);
-        else if (tb == L.tmin[s]) atomicMin(&L.smin[s], sk);
+        if (PASS == 0) {
+          const unsigned long long old = atomicMin(&L.tmin[s], tb);
+          if (tb < old) L.smin[s] = sk;
+          else if (tb == old) L.flag[s] = 1;
+        } else if (tb == L.tmin[s]) {
+          atomicMin(&L.smin[s], sk);
+        }
       }
   }
 }
@@ -393,35 +408,70 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
   __shared__ TxScatter L;
   const int lane = threadIdx.x & 63;
   const int S = g.S, nr = S * S;
-  for (int s = lane; s < nr; s += 64) L.tmin[s] = 0x7ff0000000000000ull, L.smin[s] = ~0ull;
+  for (int s = lane; s < nr; s += 64) L.tmin[s] = 0x7ff0000000000000ull, L.smin[s] = ~0ull, L.flag[s] = 0, L.slot[s] = -1;
   auto stage = [&](unsigned c0, unsigned cn) {
     __syncthreads();
-    for (unsigned l = lane; l < cn; l += 64) {
-      const int q = list[base + c0 + l];
-      L.q[l] = q;
-      L.kq[l] = pkey(key, q);
+    /* one entry per lane (cn <= 64): its polygon, key and fan count, and
+     * the first fan of each entry by a prefix sum over the wave */
+    unsigned nf = 0;
+    if ((unsigned)lane < cn) {
+      const int q = list[base + c0 + lane];
+      L.q[lane] = q;
+      L.kq[lane] = pkey(key, q);
+      nf = (unsigned)min((int)poly[(size_t)q * HCS_PREC], 8);
     }
-    if (lane == 0) {
-      /* from the list itself: the other lanes' L.q writes are not
-       * ordered before this read */
-      int acc = 0;
-      for (unsigned l = 0; l < cn; ++l) {
-        L.fan0[l] = acc;
-        acc += min((int)poly[(size_t)list[base + c0 + l] * HCS_PREC], 8);
-      }
-      L.fan0[cn] = acc;
+    unsigned incl = nf;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned t = __shfl(incl, lane >= o ? lane - o : lane, 64);
+      if (lane >= o) incl += t;
     }
+    if ((unsigned)lane < cn) L.fan0[lane] = (int)(incl - nf);
+    if ((unsigned)lane == cn - 1) L.fan0[cn] = (int)incl;
     __syncthreads();
   };
   const bool one_chunk = n <= HCS_TX_CHUNK;
-  for (int pass = 0; pass < 2; ++pass)
+  for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
+    const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
+    stage(c0, cn);
+    scatter_chunk<0>(L, poly, g, x, y, cn);
+    __syncthreads();
+  }
+  /* settle the rays whose owner holds the least t (one chunk: the owner's
+   * polygon is staged); flag the rest for the second pass */
+  bool again = false;
+  for (int s = lane; s < nr; s += 64) {
+    if (L.tmin[s] == 0x7ff0000000000000ull) continue;
+    bool settled = false;
+    if (one_chunk && !L.flag[s]) {
+      const unsigned long long sk = L.smin[s];
+      const long long kq = (long long)(sk >> 3);
+      int bl = -1;
+      for (unsigned l = 0; l < n; ++l)
+        if (L.kq[l] == kq) bl = (int)l;
+      if (bl >= 0) {
+        const double* P = poly + (size_t)L.q[bl] * HCS_PREC;
+        TxFan c;
+        if (fan_setup(P, min((int)P[0], 8), (int)(sk & 7), g, x, y, c)) {
+          const int i = s / g.S, j = s % g.S;
+          const double t = fan_hit(c, g, x, y, i, j);
+          settled = t > 0.0 && (unsigned long long)__double_as_longlong(t) == L.tmin[s];
+        }
+      }
+      if (settled) L.slot[s] = bl;
+    }
+    if (!settled) L.flag[s] = 1, L.smin[s] = ~0ull, again = true;
+  }
+  if (__any(again)) {
+    __syncthreads();
     for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
       const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
-      if (!one_chunk || (pass == 0 && c0 == 0)) stage(c0, cn);
-      if (pass == 0) scatter_chunk<0>(L, poly, g, x, y, cn);
-      else scatter_chunk<1>(L, poly, g, x, y, cn);
+      if (!one_chunk) stage(c0, cn);
+      scatter_chunk<1>(L, poly, g, x, y, cn);
       __syncthreads();
     }
+  }
+  __syncthreads();
   /* the chosen triangle of every ray, evaluated as in the ray-major loop */
   const double rSR = g.res / S;
   const double rmean = 1. / (S * S);
@@ -441,13 +491,14 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
     const unsigned long long sk = active ? L.smin[s] : ~0ull;
     const long long kq = (long long)(sk >> 3);
     const int bk = (int)(sk & 7);
-    /* the polygon of that key: a search of the list (restaged per chunk
-     * when the list has more than one) */
-    int bq = -1;
+    /* the polygon of that key: the settled ray's slot, else a search of the
+     * list (restaged per chunk when the list has more than one) */
+    const int sl = active ? L.slot[s] : -1;
+    int bq = sl >= 0 ? L.q[sl] : -1;
     for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
       const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
       if (!one_chunk) stage(c0, cn);
-      if (tb != 0x7ff0000000000000ull)
+      if (tb != 0x7ff0000000000000ull && sl < 0)
         for (unsigned l = 0; l < cn; ++l)
           if (L.kq[l] == kq) bq = L.q[l];
     }

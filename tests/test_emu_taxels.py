@@ -68,12 +68,21 @@ def _run(emu, P, K, X, g, S_, window, sigma, mode):
 
 
 @pytest.mark.parametrize("case,window,sigma", [("pad4", 0, 0.0), ("pad4", 1, 0.3), ("pad4", 2, 0.5), ("pad4", 3, 0.0),
-                                               ("pad4_moved", 0, 0.0), ("pad2", 0, 0.0)])
+                                               ("pad4_moved", 0, 0.0), ("pad2", 0, 0.0), ("pad4_doubled", 0, 0.0)])
 def test_fan_major_sampler_equals_ray_major_and_oracle(emu, case, window, sigma):
+    """pad4_doubled lists every polygon twice under a second key: every hit
+    ties with its copy (the tie rule picks the smaller key, the value is the
+    single copy's), and contact taxels hold more than one chunk of 64
+    polygons, so the fan-major sampler's second pass and its multi-chunk
+    path run."""
     sc = S.differentiation_env(0, 1) if case != "pad2" else S.tactile_pad(0)
     if case.endswith("moved"):
         sc = _moved(sc, 3)
     P, K = _polys(sc)
+    if case.endswith("doubled"):
+        K2 = K.copy()
+        K2[:, 0] = 1
+        P, K = np.ascontiguousarray(np.concatenate([P, P])), np.ascontiguousarray(np.concatenate([K, K2]))
     g = sc["grid"]
     S_ = g["sampling_resolution"]
     a = _run(emu, P, K, sc["X_WB"], g, S_, window, sigma, 0)
