@@ -292,8 +292,10 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
 #define HCS_TX_RAYS 576
 struct TxScatter {
   unsigned long long tmin[HCS_TX_RAYS], smin[HCS_TX_RAYS];
-  int flag[HCS_TX_RAYS]; /* the ray needs the second pass */
-  int slot[HCS_TX_RAYS]; /* a settled ray: the list slot of its polygon (else -1) */
+  /* first pass: list slot * 8 + fan of the hit that lowered t last; after
+   * the check, that of a settled ray, -1 for the others */
+  int own[HCS_TX_RAYS];
+  unsigned char flag[HCS_TX_RAYS]; /* the ray needs the second pass */
   int q[HCS_TX_CHUNK], fan0[HCS_TX_CHUNK + 1];
   long long kq[HCS_TX_CHUNK];
 };
@@ -303,7 +305,9 @@ struct TxScatter {
  * (|a| < 1e-10, which the hit test rejects, or an empty range). */
 struct TxFan {
   double v0[3], e1[3], e2[3], h[3], f;
+  double px[3], py[3], m; /* the fan in the sensor's (x, y), the range margin */
   int i0, i1, j0, j1;
+  bool rows; /* per-row ranges apply (false: the fan gets every ray) */
 };
 __device__ inline bool fan_setup(const double* P, int nv, int k, const TaxelGridDev& g, int x, int y, TxFan& c) {
   const int S = g.S;
@@ -320,6 +324,7 @@ __device__ inline bool fan_setup(const double* P, int nv, int k, const TaxelGrid
   if (fabs(a) < 1e-10) return false;
   c.f = 1.0 / a;
   const double L2 = fmax(dot3(c.e1, c.e1), dot3(c.e2, c.e2));
+  c.rows = false;
   if (fabs(a) < 1e-3 * L2) {
     c.i0 = c.j0 = 0, c.i1 = c.j1 = S - 1;
     return true;
@@ -333,15 +338,57 @@ __device__ inline bool fan_setup(const double* P, int nv, int k, const TaxelGrid
     const double px = (g.X[0] * d[0] + g.X[3] * d[1]) + g.X[6] * d[2];
     const double py = (g.X[1] * d[0] + g.X[4] * d[1]) + g.X[7] * d[2];
     const double pz = (g.X[2] * d[0] + g.X[5] * d[1]) + g.X[8] * d[2];
+    c.px[t] = px, c.py[t] = py;
     lx = fmin(lx, px), hx = fmax(hx, px), ly = fmin(ly, py), hy = fmax(hy, py);
     gap = fmax(gap, fabs(oz - pz));
   }
   const double m = 1e-9 * (1.0 + fmax(hx - lx, hy - ly) + gap);
+  c.m = m;
+  c.rows = true;
   const double fi0 = fmax(floor((lx - m - ox) / rSR), 0.0), fi1 = fmin(ceil((hx + m - ox) / rSR), S - 1.0);
   const double fj0 = fmax(floor((ly - m - oy) / rSR), 0.0), fj1 = fmin(ceil((hy + m - oy) / rSR), S - 1.0);
   if (!(fi0 <= fi1) || !(fj0 <= fj1)) return false;
   c.i0 = (int)fi0, c.i1 = (int)fi1, c.j0 = (int)fj0, c.j1 = (int)fj1;
   return true;
+}
+
+/* The rays of row i (ray x index) that fan c can hold: the fan's y extent
+ * over the slab of x within 2m of the row's x (each edge cut to the slab),
+ * widened by 2m, within the fan's range [j0, j1]; empty when ja > jb.  The
+ * margin m is the fan's range margin, so a row range holds every ray the
+ * range holds that the hit test can accept. */
+__device__ inline void fan_row(const TxFan& c, const TaxelGridDev& g, int x, int y, int i, int& ja, int& jb) {
+  ja = c.j0, jb = c.j1;
+  if (!c.rows) return;
+  const double rSR = g.res / g.S;
+  const double ox = -g.xs + x * g.res + 0.5 * rSR, oy = -g.ys + y * g.res + 0.5 * rSR;
+  const double xa = ox + i * rSR - 2.0 * c.m, xb = ox + i * rSR + 2.0 * c.m;
+  double lo = 1e300, hi = -1e300;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int e1 = e == 2 ? 0 : e + 1;
+    const double x0 = c.px[e], y0 = c.py[e], x1 = c.px[e1], y1 = c.py[e1];
+    if (fmax(x0, x1) < xa || fmin(x0, x1) > xb) continue;
+    const double dx = x1 - x0;
+    double ta = 0.0, tb = 1.0;
+    if (dx != 0.0) {
+      ta = (xa - x0) / dx, tb = (xb - x0) / dx;
+      if (ta > tb) {
+        const double t = ta;
+        ta = tb, tb = t;
+      }
+      ta = fmin(fmax(ta, 0.0), 1.0), tb = fmin(fmax(tb, 0.0), 1.0);
+    }
+    const double ya = y0 + ta * (y1 - y0), yb = y0 + tb * (y1 - y0);
+    lo = fmin(lo, fmin(ya, yb)), hi = fmax(hi, fmax(ya, yb));
+  }
+  if (!(lo <= hi)) {
+    ja = 1, jb = 0;
+    return;
+  }
+  const double fa = floor((lo - 2.0 * c.m - oy) / rSR), fb = ceil((hi + 2.0 * c.m - oy) / rSR);
+  ja = fa > (double)ja ? (int)fmin(fa, (double)c.j1 + 1.0) : ja;
+  jb = fb < (double)jb ? (int)fmax(fb, (double)c.j0 - 1.0) : jb;
 }
 
 /* The hit of ray (i, j) with fan c, as the ray-major loop computes it
@@ -384,8 +431,10 @@ __device__ inline void scatter_chunk(TxScatter& L, const double* __restrict__ po
     TxFan c;
     if (!fan_setup(P, nv, k, g, x, y, c)) continue;
     const unsigned long long sk = ((unsigned long long)L.kq[l] << 3) | (unsigned)k;
-    for (int i = c.i0; i <= c.i1; ++i)
-      for (int j = c.j0; j <= c.j1; ++j) {
+    for (int i = c.i0; i <= c.i1; ++i) {
+      int ja, jb;
+      fan_row(c, g, x, y, i, ja, jb);
+      for (int j = ja; j <= jb; ++j) {
         const int s = i * g.S + j;
         if (PASS == 1 && !L.flag[s]) continue;
         const double t = fan_hit(c, g, x, y, i, j);
@@ -393,12 +442,13 @@ __device__ inline void scatter_chunk(TxScatter& L, const double* __restrict__ po
         const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
         if (PASS == 0) {
           const unsigned long long old = atomicMin(&L.tmin[s], tb);
-          if (tb < old) L.smin[s] = sk;
+          if (tb < old) L.own[s] = (l << 3) | k;
           else if (tb == old) L.flag[s] = 1;
         } else if (tb == L.tmin[s]) {
           atomicMin(&L.smin[s], sk);
         }
       }
+    }
   }
 }
 
@@ -408,7 +458,7 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
   __shared__ TxScatter L;
   const int lane = threadIdx.x & 63;
   const int S = g.S, nr = S * S;
-  for (int s = lane; s < nr; s += 64) L.tmin[s] = 0x7ff0000000000000ull, L.smin[s] = ~0ull, L.flag[s] = 0, L.slot[s] = -1;
+  for (int s = lane; s < nr; s += 64) L.tmin[s] = 0x7ff0000000000000ull, L.smin[s] = ~0ull, L.flag[s] = 0, L.own[s] = -1;
   auto stage = [&](unsigned c0, unsigned cn) {
     __syncthreads();
     /* one entry per lane (cn <= 64): its polygon, key and fan count, and
@@ -444,23 +494,16 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
     if (L.tmin[s] == 0x7ff0000000000000ull) continue;
     bool settled = false;
     if (one_chunk && !L.flag[s]) {
-      const unsigned long long sk = L.smin[s];
-      const long long kq = (long long)(sk >> 3);
-      int bl = -1;
-      for (unsigned l = 0; l < n; ++l)
-        if (L.kq[l] == kq) bl = (int)l;
-      if (bl >= 0) {
-        const double* P = poly + (size_t)L.q[bl] * HCS_PREC;
-        TxFan c;
-        if (fan_setup(P, min((int)P[0], 8), (int)(sk & 7), g, x, y, c)) {
-          const int i = s / g.S, j = s % g.S;
-          const double t = fan_hit(c, g, x, y, i, j);
-          settled = t > 0.0 && (unsigned long long)__double_as_longlong(t) == L.tmin[s];
-        }
+      const int bl = L.own[s] >> 3, bk = L.own[s] & 7;
+      const double* P = poly + (size_t)L.q[bl] * HCS_PREC;
+      TxFan c;
+      if (fan_setup(P, min((int)P[0], 8), bk, g, x, y, c)) {
+        const int i = s / g.S, j = s % g.S;
+        const double t = fan_hit(c, g, x, y, i, j);
+        settled = t > 0.0 && (unsigned long long)__double_as_longlong(t) == L.tmin[s];
       }
-      if (settled) L.slot[s] = bl;
     }
-    if (!settled) L.flag[s] = 1, L.smin[s] = ~0ull, again = true;
+    if (!settled) L.flag[s] = 1, L.own[s] = -1, again = true;
   }
   if (__any(again)) {
     __syncthreads();
@@ -488,12 +531,14 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
     const bool active = s < nr;
     const unsigned long long tb = active ? L.tmin[s] : 0x7ff0000000000000ull;
     const double best_t = __longlong_as_double((long long)tb);
+    /* a settled ray's owner, else the least key of the second pass */
+    const int ow = active ? L.own[s] : -1;
     const unsigned long long sk = active ? L.smin[s] : ~0ull;
     const long long kq = (long long)(sk >> 3);
-    const int bk = (int)(sk & 7);
-    /* the polygon of that key: the settled ray's slot, else a search of the
-     * list (restaged per chunk when the list has more than one) */
-    const int sl = active ? L.slot[s] : -1;
+    const int bk = ow >= 0 ? (ow & 7) : (int)(sk & 7);
+    /* the polygon: the settled ray's list slot, else a search of the list
+     * for the key (restaged per chunk when the list has more than one) */
+    const int sl = ow >= 0 ? ow >> 3 : -1;
     int bq = sl >= 0 ? L.q[sl] : -1;
     for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
       const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
