@@ -685,14 +685,29 @@ __device__ inline bool survives(const EnvDesc& E, const double* X, int a, int b,
   double off = -((cb - cA) / mag);
   if (!(((nh[0] * r[20] + nh[1] * r[21]) + nh[2] * r[22]) > HCS_COS_ALPHA)) return false;
   if (!(((-nh[0] * gbh[0] + -nh[1] * gbh[1]) + -nh[2] * gbh[2]) > HCS_COS_ALPHA)) return false;
-  const int32_t* ta = E.TA + 4 * (size_t)a;
-  int above = 0;
+  /* the slice of tet a by the plane (false: the plane misses it), then the
+   * face-plane test of the triangle case on the slice against tet b's
+   * half-spaces, placed in A's frame as process_candidate places them for
+   * the clip */
+  Poly<double> p;
+  if (!slice_tet<double>(E, a, nh, off, p)) return false;
+  double vmax = 0.0;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double* v = E.VA + 3 * (size_t)ta[i];
-    above += (dot3(nh, v) + off) > 0;
+  for (int i = 0; i < 4; ++i) vmax = fmax(vmax, fmax(fabs(p.x[i]), fmax(fabs(p.y[i]), fabs(p.z[i]))));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double nb[3] = {rb[3 * k], rb[3 * k + 1], rb[3 * k + 2]};
+    double pn[3];
+    rot(X, nb, pn);
+    const double po = rb[12 + k] - dot3(pn, X + 9);
+    const double tol = HCS_ULP32 * (3.0 * vmax + fabs(po)) + 1e-15;
+    bool out = true;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      out = out && (i >= p.n || ((pn[0] * p.x[i] + pn[1] * p.y[i]) + pn[2] * p.z[i]) + po > tol);
+    if (out) return false;
   }
-  return above != 0 && above != 4;
+  return true;
 }
 
 /* Whether an item can have a candidate: box P (B's frame) placed in A's
