@@ -292,8 +292,8 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
 #define HCS_TX_RAYS 576
 struct TxScatter {
   unsigned long long tmin[HCS_TX_RAYS], smin[HCS_TX_RAYS];
-  /* first pass: list slot * 8 + fan of the hit that lowered t last; after
-   * the check, that of a settled ray, -1 for the others */
+  /* first pass: list position * 8 + fan of the hit that lowered t last;
+   * after the check, that of a settled ray, -1 for the others */
   int own[HCS_TX_RAYS];
   unsigned char flag[HCS_TX_RAYS]; /* the ray needs the second pass */
   int q[HCS_TX_CHUNK], fan0[HCS_TX_CHUNK + 1];
@@ -391,6 +391,11 @@ __device__ inline void fan_row(const TxFan& c, const TaxelGridDev& g, int x, int
   jb = fb < (double)jb ? (int)fmax(fb, (double)c.j0 - 1.0) : jb;
 }
 
+/* The hit of ray (i, j) with fan k of polygon P without the fan's ray
+ * range: fan_setup's hit constants, then fan_hit (the same arithmetic). */
+__device__ inline double fan_hit_at(const double* P, int nv, int k, const TaxelGridDev& g, int x, int y, int i,
+                                    int j);
+
 /* The hit of ray (i, j) with fan c, as the ray-major loop computes it
  * (origin R loc + p + 1e-8 n, direction -n): t > 0, or a negative value. */
 __device__ inline double fan_hit(const TxFan& c, const TaxelGridDev& g, int x, int y, int i, int j) {
@@ -415,11 +420,28 @@ __device__ inline double fan_hit(const TxFan& c, const TaxelGridDev& g, int x, i
   return t > 0.0 ? t : -1.0;
 }
 
+__device__ inline double fan_hit_at(const double* P, int nv, int k, const TaxelGridDev& g, int x, int y, int i,
+                                    int j) {
+  const double Dv[3] = {-g.X[2], -g.X[5], -g.X[8]};
+  const int k1 = (k + 1 == nv) ? 0 : k + 1;
+  const double* v0 = P + 1 + 3 * k;
+  const double* v1 = P + 1 + 3 * k1;
+  const double* v2 = P + 33;
+  TxFan c;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) c.v0[d] = v0[d], c.e1[d] = v1[d] - v0[d], c.e2[d] = v2[d] - v0[d];
+  cross3(Dv, c.e2, c.h);
+  const double a = dot3(c.e1, c.h);
+  if (fabs(a) < 1e-10) return -1.0;
+  c.f = 1.0 / a;
+  return fan_hit(c, g, x, y, i, j);
+}
+
 /* One pass over the fans of list[base + c0 ..] for the wave: PASS 0 keeps
  * the least t per ray, PASS 1 the least (key, fan) among hits at that t. */
 template <int PASS>
 __device__ inline void scatter_chunk(TxScatter& L, const double* __restrict__ poly, const TaxelGridDev& g, int x,
-                                     int y, unsigned cn) {
+                                     int y, unsigned cn, unsigned c0) {
   const int lane = threadIdx.x & 63;
   const int nf = L.fan0[cn];
   for (int f = lane; f < nf; f += 64) {
@@ -442,7 +464,7 @@ __device__ inline void scatter_chunk(TxScatter& L, const double* __restrict__ po
         const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
         if (PASS == 0) {
           const unsigned long long old = atomicMin(&L.tmin[s], tb);
-          if (tb < old) L.own[s] = (l << 3) | k;
+          if (tb < old) L.own[s] = (int)(((c0 + (unsigned)l) << 3) | (unsigned)k);
           else if (tb == old) L.flag[s] = 1;
         } else if (tb == L.tmin[s]) {
           atomicMin(&L.smin[s], sk);
@@ -484,24 +506,20 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
   for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
     const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
     stage(c0, cn);
-    scatter_chunk<0>(L, poly, g, x, y, cn);
+    scatter_chunk<0>(L, poly, g, x, y, cn, c0);
     __syncthreads();
   }
-  /* settle the rays whose owner holds the least t (one chunk: the owner's
-   * polygon is staged); flag the rest for the second pass */
+  /* settle the rays whose owner holds the least t; flag the rest for the
+   * second pass */
   bool again = false;
   for (int s = lane; s < nr; s += 64) {
     if (L.tmin[s] == 0x7ff0000000000000ull) continue;
     bool settled = false;
-    if (one_chunk && !L.flag[s]) {
-      const int bl = L.own[s] >> 3, bk = L.own[s] & 7;
-      const double* P = poly + (size_t)L.q[bl] * HCS_PREC;
-      TxFan c;
-      if (fan_setup(P, min((int)P[0], 8), bk, g, x, y, c)) {
-        const int i = s / g.S, j = s % g.S;
-        const double t = fan_hit(c, g, x, y, i, j);
-        settled = t > 0.0 && (unsigned long long)__double_as_longlong(t) == L.tmin[s];
-      }
+    if (!L.flag[s]) {
+      const int pos = L.own[s] >> 3, bk = L.own[s] & 7;
+      const double* P = poly + (size_t)(one_chunk ? L.q[pos] : list[base + pos]) * HCS_PREC;
+      const double t = fan_hit_at(P, min((int)P[0], 8), bk, g, x, y, s / g.S, s % g.S);
+      settled = t > 0.0 && (unsigned long long)__double_as_longlong(t) == L.tmin[s];
     }
     if (!settled) L.flag[s] = 1, L.own[s] = -1, again = true;
   }
@@ -510,7 +528,7 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
     for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
       const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
       if (!one_chunk) stage(c0, cn);
-      scatter_chunk<1>(L, poly, g, x, y, cn);
+      scatter_chunk<1>(L, poly, g, x, y, cn, c0);
       __syncthreads();
     }
   }
@@ -536,17 +554,20 @@ __device__ inline double sample_cell_scatter(const double* __restrict__ poly, co
     const unsigned long long sk = active ? L.smin[s] : ~0ull;
     const long long kq = (long long)(sk >> 3);
     const int bk = ow >= 0 ? (ow & 7) : (int)(sk & 7);
-    /* the polygon: the settled ray's list slot, else a search of the list
-     * for the key (restaged per chunk when the list has more than one) */
-    const int sl = ow >= 0 ? ow >> 3 : -1;
-    int bq = sl >= 0 ? L.q[sl] : -1;
-    for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
-      const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
-      if (!one_chunk) stage(c0, cn);
-      if (tb != 0x7ff0000000000000ull && sl < 0)
-        for (unsigned l = 0; l < cn; ++l)
-          if (L.kq[l] == kq) bq = L.q[l];
-    }
+    /* the polygon: the settled ray's list position, else a search of the
+     * list for the key (restaged per chunk when the list has more than one,
+     * and only when some lane needs it) */
+    const int pos = ow >= 0 ? ow >> 3 : -1;
+    int bq = pos < 0 ? -1 : (one_chunk ? L.q[pos] : list[base + pos]);
+    const bool search = tb != 0x7ff0000000000000ull && pos < 0;
+    if (__any(search))
+      for (unsigned c0 = 0; c0 < n; c0 += HCS_TX_CHUNK) {
+        const unsigned cn = (n - c0) < HCS_TX_CHUNK ? (n - c0) : HCS_TX_CHUNK;
+        if (!one_chunk) stage(c0, cn);
+        if (search)
+          for (unsigned l = 0; l < cn; ++l)
+            if (L.kq[l] == kq) bq = L.q[l];
+      }
     if (!active) continue;
     const int i = s / S, j = s % S;
     if (bq >= 0 && best_t < 1.5 * g.zs && best_t > 0.0) {
