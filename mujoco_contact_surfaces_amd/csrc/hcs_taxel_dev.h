@@ -445,8 +445,14 @@ __device__ inline void scatter_chunk(TxScatter& L, const double* __restrict__ po
   const int lane = threadIdx.x & 63;
   const int nf = L.fan0[cn];
   for (int f = lane; f < nf; f += 64) {
+    /* the entry of fan f: the last entry whose first fan is at most f
+     * (binary search; every entry has at least one fan) */
     int l = 0;
-    while (L.fan0[l + 1] <= f) ++l;
+    for (int hi = (int)cn - 1; l < hi;) {
+      const int mid = (l + hi + 1) >> 1;
+      if (L.fan0[mid] <= f) l = mid;
+      else hi = mid - 1;
+    }
     const int k = f - L.fan0[l];
     const double* P = poly + (size_t)L.q[l] * HCS_PREC;
     const int nv = min((int)P[0], 8);
