@@ -420,36 +420,55 @@ struct Acc {
   }
 };
 
-/* plugin.cpp:345-404 (one face) followed by plugin.cpp:435-482 (its force);
- * returns true when a point collision is emitted.  nW: polygon normal out of
- * B into A; gAW/gBW: field gradients in W (gBW unused when B is rigid).
- * rec (optional) receives p[3], n[3], fn0, k, damping, area, p0, g. */
+/* The quantities of plugin.cpp:345-404 that depend only on the polygon (its
+ * normal and the two field gradients), shared by the polygon's faces:
+ * n out of N into M, and the effective gradient g = 1/(1/gM + 1/gN) along
+ * it.  ok is false when either side's gradient along n is below 1e-14 (no
+ * face of the polygon is emitted).  nW: polygon normal out of B into A;
+ * gAW/gBW: field gradients in W (gBW unused when B is rigid). */
 template <class T>
-__device__ inline bool eval_face(const PairW<T>& pc, T area, const T* cW, T p0, const T* nW, const T* gAW,
-                                 const T* gBW, Acc<T>& acc, double* rec) {
-  if (!(val(area) > 1.0e-14)) return false;
+struct FaceConst {
+  T n[3], g;
+  bool ok;
+};
+template <class T>
+__device__ inline void face_const(const PairW<T>& pc, const T* nW, const T* gAW, const T* gBW, FaceConst<T>& fc) {
   /* M / N quantities picked by value (a pointer picked between two local
    * arrays would keep both in private memory) */
   const bool aM = pc.a_is_M;
-  T n[3], gM[3], gN[3];
+  T gM[3], gN[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    n[c] = aM ? nW[c] : -nW[c];
+    fc.n[c] = aM ? nW[c] : -nW[c];
     gM[c] = aM ? gAW[c] : gBW[c];
     gN[c] = aM ? gBW[c] : gAW[c];
   }
   const bool M_soft = aM ? true : (bool)pc.b_soft, N_soft = aM ? (bool)pc.b_soft : true;
   const double kInf = __builtin_inf();
-  const T gMv = M_soft ? dot3(gM, n) : T(kInf);
-  const T gNv = N_soft ? -dot3(gN, n) : T(kInf);
-  if (val(gMv) < 1.0e-14 || val(gNv) < 1.0e-14) return false;
+  const T gMv = M_soft ? dot3(gM, fc.n) : T(kInf);
+  const T gNv = N_soft ? -dot3(gN, fc.n) : T(kInf);
+  fc.ok = !(val(gMv) < 1.0e-14 || val(gNv) < 1.0e-14);
   /* 1/(1/gM + 1/gN); the rigid side contributes exactly 0 (value and tangent). */
   T inv = T(0.0);
-  if (M_soft) inv = inv + T(1.0) / gMv;
-  if (N_soft) inv = inv + T(1.0) / gNv;
-  const T g = T(1.0) / inv;
+  if (fc.ok && M_soft) inv = inv + T(1.0) / gMv;
+  if (fc.ok && N_soft) inv = inv + T(1.0) / gNv;
+  fc.g = fc.ok ? T(1.0) / inv : T(0.0);
+}
+
+/* plugin.cpp:345-404 (one face) followed by plugin.cpp:435-482 (its force),
+ * with the polygon's constants fc (face_const); returns true when a point
+ * collision is emitted.  rec (optional) receives p[3], n[3], fn0, k,
+ * damping, area, p0, g. */
+template <class T>
+__device__ inline bool eval_face(const PairW<T>& pc, const FaceConst<T>& fc, T area, const T* cW, T p0, Acc<T>& acc,
+                                 double* rec) {
+  if (!(val(area) > 1.0e-14)) return false;
+  if (!fc.ok) return false;
+  const T* n = fc.n;
+  const T g = fc.g;
   const T fn0 = area * p0;
   const T k = area * g;
+  const bool aM = pc.a_is_M;
   T oM[3], oN[3], VM[6], VN[6];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {

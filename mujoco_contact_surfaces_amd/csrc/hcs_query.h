@@ -130,6 +130,8 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
   T nW[3], gAW[3];
   rot(ic.pw.X_WA, nA, nW);
   rot(ic.pw.X_WA, gA, gAW);
+  FaceConst<T> fc;
+  face_const(ic.pw, nW, gAW, gBW, fc);
   const int n = p.n;
   if (o && o->poly_rec && (!o->poly_env || o->poly_env[env])) {
     /* the lanes that keep a polygon here reserve their record slots with
@@ -204,7 +206,7 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
         xform(ic.pw.X_WA, cS, cW);
         double rec[HCS_FREC];
         bool want = o && o->face_rec;
-        if (eval_face(ic.pw, area, cW, p0, nW, gAW, gBW, acc, rec)) {
+        if (eval_face(ic.pw, fc, area, cW, p0, acc, rec)) {
           ++nf;
           if (want) {
             unsigned long long slot = atomicAdd(o->face_cnt, 1ull);
@@ -224,7 +226,7 @@ __device__ inline void emit_polygon(const ItemCtx<T>& ic, const Poly<T>& p, cons
     xform(ic.pw.X_WA, C, cW);
     double rec[HCS_FREC];
     bool want = o && o->face_rec;
-    if (eval_face(ic.pw, area, cW, pC, nW, gAW, gBW, acc, rec)) {
+    if (eval_face(ic.pw, fc, area, cW, pC, acc, rec)) {
       ++nf;
       if (want) {
         unsigned long long slot = atomicAdd(o->face_cnt, 1ull);
