@@ -532,21 +532,27 @@ __global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 2) void k_cand(QueryArgs A
     /* persistent: the block's own queue first, then the others.  The next
      * item's record is loaded, and the work counter advanced for the one
      * after, while the current item runs, so neither round trip is on an
-     * item's critical path. */
+     * item's critical path.  With one queue, block b's first item is item
+     * b and the counter hands out the items from gridDim.x on, so the
+     * launch does not start with one atomic per wave on a single address. */
     const int x0 = (int)(blockIdx.x % nq);
+    const unsigned first = nq == 1 ? gridDim.x : 0u;
     for (int k = 0; k < nq; ++k) {
       const int x = (x0 + k) % nq;
       const unsigned n = A.live_cnt[x];
       const int base = A.qseg[x];
       unsigned raw = 0;
-      if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
-      unsigned v = __shfl(raw, 0, 64);
+      unsigned v = blockIdx.x;
+      if (nq != 1) {
+        if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
+        v = __shfl(raw, 0, 64);
+      }
       if (v >= n) continue;
       int idx = A.live[base + v];
       int4 rec = A.live_item[base + v];
       if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
       for (;;) {
-        const unsigned vn = __shfl(raw, 0, 64);
+        const unsigned vn = __shfl(raw, 0, 64) + first;
         int nidx = 0;
         int4 nrec = make_int4(0, 0, 0, 0);
         if (vn < n) {
