@@ -259,8 +259,7 @@ int hcs_batch_tune(hcs_batch* batch, int32_t reps, double* best_ms);
  * candidate kernel (1, or up to 8: one per XCD), work order of the items
  * (0: decreasing candidate count; 1: pairs by decreasing total count, each
  * pair's items together; 2: decreasing shader cycles of one measured
- * step; 3: pairs by decreasing count of their largest item, each pair's
- * items together)]; 6 int32. */
+ * step)]; 6 int32. */
 int hcs_batch_config(hcs_batch* batch, int32_t* cfg);
 int hcs_batch_set_config(hcs_batch* batch, const int32_t* cfg);
 /* Per-replay step times: reps graph replays back to back on hcs_stream with

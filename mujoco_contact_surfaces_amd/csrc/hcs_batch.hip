@@ -157,13 +157,10 @@ static int batch_measure_cycles(hcs_batch* b, std::vector<unsigned long long>& c
   return HCS_OK;
 }
 
-/* Work order of the items for the candidate kernel (order_mode): 0,
- * decreasing measured candidate count (longest first); 1, pairs by
- * decreasing total count with each pair's items together by decreasing
- * count; 2, decreasing measured shader cycles; 3, pairs by decreasing
- * count of their largest item (then total), each pair's items together by
- * decreasing count, so the items that run last are small ones while waves
- * that run together still share a few pairs' meshes.  Ties by item index. */
+/* Work order of the items for the candidate kernel (order_mode): decreasing
+ * measured candidate count (longest first); pairs by decreasing total count
+ * with each pair's items together by decreasing count; or decreasing
+ * measured shader cycles.  Ties by item index. */
 static int batch_order(hcs_batch* b) {
   std::vector<int> nc;
   int rc = batch_measure(b, nc);
@@ -173,20 +170,12 @@ static int batch_order(hcs_batch* b) {
   std::vector<int>& o = b->plan.order;
   o.resize(nc.size());
   for (size_t i = 0; i < o.size(); ++i) o[i] = (int)i;
-  if (b->order_mode == 1 || b->order_mode == 3) {
-    std::vector<int64_t> tot(b->npairs, 0), big(b->npairs, 0);
-    for (size_t i = 0; i < nc.size(); ++i) {
-      const int e = b->plan.items[i].env;
-      tot[e] += nc[i];
-      big[e] = std::max<int64_t>(big[e], nc[i]);
-    }
-    if (b->order_mode == 1) big.assign(b->npairs, 0);
+  if (b->order_mode == 1) {
+    std::vector<int64_t> tot(b->npairs, 0);
+    for (size_t i = 0; i < nc.size(); ++i) tot[b->plan.items[i].env] += nc[i];
     std::stable_sort(o.begin(), o.end(), [&](int x, int y) {
       const int ex = b->plan.items[x].env, ey = b->plan.items[y].env;
-      if (ex != ey) {
-        if (big[ex] != big[ey]) return big[ex] > big[ey];
-        return tot[ex] != tot[ey] ? tot[ex] > tot[ey] : ex < ey;
-      }
+      if (ex != ey) return tot[ex] != tot[ey] ? tot[ex] > tot[ey] : ex < ey;
       return nc[x] > nc[y];
     });
   } else if (b->order_mode == 2) {
@@ -450,7 +439,7 @@ int hcs_batch_config(hcs_batch* b, int32_t* cfg) {
 
 int hcs_batch_set_config(hcs_batch* b, const int32_t* cfg) {
   if (!b || !cfg || cfg[0] < 1 || cfg[0] > 64 || cfg[1] < 1 || cfg[1] > 16 || cfg[2] < 0 || cfg[3] < 0 ||
-      cfg[4] < 1 || cfg[4] > HCS_QUEUES || cfg[5] < 0 || cfg[5] > 3)
+      cfg[4] < 1 || cfg[4] > HCS_QUEUES || cfg[5] < 0 || cfg[5] > 2)
     return HCS_ERR_ARG;
   /* an item holds one query per lane of the candidate kernel's wave */
   if (cfg[1] != 1)
@@ -522,7 +511,7 @@ int hcs_batch_tune(hcs_batch* b, int32_t reps, double* best_ms) {
   /* work orders by measured counts only: the order by measured cycles
    * depends on the timing of the measuring step, so its plan would differ
    * from the one timed here */
-  for (int om : {0, 1, 3}) {
+  for (int om = 0; om < 2; ++om) {
     if (om == bo) continue;
     b->order_mode = om;
     int rc = try_plan(near.data(), (int)near.size());
