@@ -629,12 +629,19 @@ __global__ __launch_bounds__(64) void k_env_reduce(const EnvDesc* __restrict__ e
   const Contrib* cb = contrib + (int64_t)d * stride;
   double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
   int64_t np = 0, nf = 0, nc = 0;
-  for (unsigned i = s0 + lane; i < s1; i += 64) {
-    const Contrib& q = cb[i];
+  auto add = [&](const Contrib& q) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) F[c] = F[c] + q.F[c], tA[c] = tA[c] + q.tA[c], tB[c] = tB[c] + q.tB[c];
     np += q.np, nf += q.nf;
+  };
+  /* lane-strided in slot order; four slots loaded before they are added,
+   * so four loads are in flight per lane */
+  unsigned i = s0 + lane;
+  for (; i < s1 && s1 - i > 192; i += 256) {
+    const Contrib q0 = cb[i], q1 = cb[i + 64], q2 = cb[i + 128], q3 = cb[i + 192];
+    add(q0), add(q1), add(q2), add(q3);
   }
+  for (; i < s1; i += 64) add(cb[i]);
   for (int i = i0 + lane; i < i1; i += 64) nc += item_ncand[i];
 #pragma unroll
   for (int c = 0; c < 3; ++c) F[c] = wave_sum(F[c]), tA[c] = wave_sum(tA[c]), tB[c] = wave_sum(tB[c]);
@@ -659,11 +666,18 @@ __global__ void k_env_combine(const int2* __restrict__ ptab, int nred, const Red
   const RedPart* p = part + (int64_t)d * nred + pe.x;
   double F[3] = {0, 0, 0}, tA[3] = {0, 0, 0}, tB[3] = {0, 0, 0};
   int64_t np = 0, nf = 0, nc = 0;
-  for (int j = 0; j < pe.y; ++j) {
+  auto add = [&](const RedPart& q) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) F[c] = F[c] + p[j].F[c], tA[c] = tA[c] + p[j].tA[c], tB[c] = tB[c] + p[j].tB[c];
-    np += p[j].np, nf += p[j].nf, nc += p[j].nc;
+    for (int c = 0; c < 3; ++c) F[c] = F[c] + q.F[c], tA[c] = tA[c] + q.tA[c], tB[c] = tB[c] + q.tB[c];
+    np += q.np, nf += q.nf, nc += q.nc;
+  };
+  /* in slice order; four slices loaded before they are added */
+  int j = 0;
+  for (; j + 3 < pe.y; j += 4) {
+    const RedPart q0 = p[j], q1 = p[j + 1], q2 = p[j + 2], q3 = p[j + 3];
+    add(q0), add(q1), add(q2), add(q3);
   }
+  for (; j < pe.y; ++j) add(p[j]);
   const bool over = (int64_t)*surv_cnt > surv_cap;
   PairOut r;
 #pragma unroll
