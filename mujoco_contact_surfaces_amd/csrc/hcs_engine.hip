@@ -528,31 +528,60 @@ __global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 2) void k_cand(QueryArgs A
       }
       v -= n;
     }
+  } else if (A.live && nq == 1) {
+    /* persistent, one queue.  Block b's first two items are items b and
+     * b + gridDim.x, and the counter hands out the items from 2 gridDim.x
+     * on: every wave starts at the launch, and one atomic per wave on a
+     * single address there would hold the first items back (atomics on one
+     * address complete one after another).  From the second item on, the
+     * next item's record is loaded, and the counter advanced for the one
+     * after, while the current item runs; the first atomic of a wave comes
+     * after its first item, when the waves no longer run in step. */
+    const unsigned n = A.live_cnt[0], G = gridDim.x;
+    const int base = A.qseg[0];
+    const unsigned v = blockIdx.x;
+    if (v < n) {
+      int idx = A.live[base + v];
+      int4 rec = A.live_item[base + v];
+      unsigned vn = v + G;
+      bool counted = false;
+      for (;;) {
+        int nidx = 0;
+        int4 nrec = make_int4(0, 0, 0, 0);
+        unsigned raw = 0;
+        if (vn < n) {
+          nidx = A.live[base + vn];
+          nrec = A.live_item[base + vn];
+          if (counted && lane == 0) raw = atomicAdd(A.counter, 1u);
+        }
+        run_cand<MODE, KS>(L, A, idx, Item{rec.x, rec.y, rec.z, rec.w}, s_next, s_end);
+        if (vn >= n) break;
+        idx = nidx;
+        rec = nrec;
+        if (!counted && lane == 0) raw = atomicAdd(A.counter, 1u);
+        counted = true;
+        vn = __shfl(raw, 0, 64) + 2 * G;
+      }
+    }
   } else if (A.live) {
-    /* persistent: the block's own queue first, then the others.  The next
-     * item's record is loaded, and the work counter advanced for the one
-     * after, while the current item runs, so neither round trip is on an
-     * item's critical path.  With one queue, block b's first item is item
-     * b and the counter hands out the items from gridDim.x on, so the
-     * launch does not start with one atomic per wave on a single address. */
+    /* persistent over several queues: the block's own queue first, then
+     * the others.  The next item's record is loaded, and the work counter
+     * advanced for the one after, while the current item runs, so neither
+     * round trip is on an item's critical path. */
     const int x0 = (int)(blockIdx.x % nq);
-    const unsigned first = nq == 1 ? gridDim.x : 0u;
     for (int k = 0; k < nq; ++k) {
       const int x = (x0 + k) % nq;
       const unsigned n = A.live_cnt[x];
       const int base = A.qseg[x];
       unsigned raw = 0;
-      unsigned v = blockIdx.x;
-      if (nq != 1) {
-        if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
-        v = __shfl(raw, 0, 64);
-      }
+      if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
+      const unsigned v = __shfl(raw, 0, 64);
       if (v >= n) continue;
       int idx = A.live[base + v];
       int4 rec = A.live_item[base + v];
       if (lane == 0) raw = atomicAdd(A.counter + x, 1u);
       for (;;) {
-        const unsigned vn = __shfl(raw, 0, 64) + first;
+        const unsigned vn = __shfl(raw, 0, 64);
         int nidx = 0;
         int4 nrec = make_int4(0, 0, 0, 0);
         if (vn < n) {

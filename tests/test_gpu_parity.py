@@ -250,12 +250,15 @@ def test_batched_matches_plain_and_oracle(engine):
         if tet_max == 0:
             W_tri = W4
     assert b.config()["split_target"] == 8
-    # every work order (by count, by pair, by measured cycles) gives the
-    # same candidate counts and the same slot-ordered sums
-    say("batched orders")
-    for order in (1, 2, 0):
-        b.set_config(2, 1, 0, 8, 1, order)
-        assert b.config()["order"] == order
+    # every work order (by count, by pair, by measured cycles), one queue
+    # or one per XCD, and persistent grids of one and two blocks per CU
+    # (with one queue a block's first two items are fixed, the rest come
+    # from the counter) give the same candidate counts and the same
+    # slot-ordered sums
+    say("batched orders; live items", int((b.item_stats()[:, 9] > 0).sum()))
+    for bpc, queues, order in ((2, 1, 1), (2, 1, 2), (2, 1, 0), (1, 1, 1), (2, 8, 1)):
+        b.set_config(bpc, 1, 0, 8, queues, order)
+        assert b.config()["order"] == order and b.config()["queues"] == queues
         b.step()
         W5, cnt5 = b.get()
         np.testing.assert_array_equal(cnt5, cnt)
