@@ -529,14 +529,15 @@ __global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 2) void k_cand(QueryArgs A
       v -= n;
     }
   } else if (A.live && nq == 1) {
-    /* persistent, one queue.  Block b's first two items are items b and
-     * b + gridDim.x, and the counter hands out the items from 2 gridDim.x
-     * on: every wave starts at the launch, and one atomic per wave on a
-     * single address there would hold the first items back (atomics on one
-     * address complete one after another).  From the second item on, the
-     * next item's record is loaded, and the counter advanced for the one
-     * after, while the current item runs; the first atomic of a wave comes
-     * after its first item, when the waves no longer run in step. */
+    /* persistent, one queue.  Block b's first three items are items b,
+     * b + G and b + 2G (G = gridDim.x), and the counter hands out the
+     * items from 3G on: every wave starts at the launch, and one atomic per
+     * wave on a single address there would hold the first items back
+     * (atomics on one address complete one after another).  While an item
+     * runs, the next item's record is loaded, and from the second item on
+     * the counter is advanced for the one after; a wave's first atomic is
+     * issued after its first item, when the waves no longer run in step,
+     * and its result is needed only after the second. */
     const unsigned n = A.live_cnt[0], G = gridDim.x;
     const int base = A.qseg[0];
     const unsigned v = blockIdx.x;
@@ -544,23 +545,20 @@ __global__ __launch_bounds__(64, KS == KIND_TRI ? 3 : 2) void k_cand(QueryArgs A
       int idx = A.live[base + v];
       int4 rec = A.live_item[base + v];
       unsigned vn = v + G;
-      bool counted = false;
-      for (;;) {
+      for (int k = 0;; ++k) {
         int nidx = 0;
         int4 nrec = make_int4(0, 0, 0, 0);
         unsigned raw = 0;
         if (vn < n) {
           nidx = A.live[base + vn];
           nrec = A.live_item[base + vn];
-          if (counted && lane == 0) raw = atomicAdd(A.counter, 1u);
+          if (k > 0 && lane == 0) raw = atomicAdd(A.counter, 1u);
         }
         run_cand<MODE, KS>(L, A, idx, Item{rec.x, rec.y, rec.z, rec.w}, s_next, s_end);
         if (vn >= n) break;
         idx = nidx;
         rec = nrec;
-        if (!counted && lane == 0) raw = atomicAdd(A.counter, 1u);
-        counted = true;
-        vn = __shfl(raw, 0, 64) + 2 * G;
+        vn = k == 0 ? v + 2 * G : __shfl(raw, 0, 64) + 3 * G;
       }
     }
   } else if (A.live) {

@@ -252,7 +252,7 @@ def test_batched_matches_plain_and_oracle(engine):
     assert b.config()["split_target"] == 8
     # every work order (by count, by pair, by measured cycles), one queue
     # or one per XCD, and persistent grids of one and two blocks per CU
-    # (with one queue a block's first two items are fixed, the rest come
+    # (with one queue a block's first three items are fixed, the rest come
     # from the counter) give the same candidate counts and the same
     # slot-ordered sums
     say("batched orders; live items", int((b.item_stats()[:, 9] > 0).sum()))
