@@ -187,19 +187,35 @@ __global__ __launch_bounds__(256) void k_txb_bin(const double* __restrict__ poly
 
 /* The taxels that have candidate polygons (and whose list fits the list
  * buffer; an overflowed list is re-run after the step with a larger one),
- * appended to live[] with one atomic per wave; every other taxel is 0 (the
- * output is cleared before). */
-__global__ __launch_bounds__(256) void k_txb_live(const unsigned* __restrict__ cnt, const unsigned* __restrict__ off,
-                                                  int ncell, int64_t list_cap, int* __restrict__ live,
-                                                  unsigned* __restrict__ nlive) {
+ * appended to live[] with one atomic per block of HCS_TXB_LIVE threads
+ * (atomics on the one counter complete one after another, so one per wave
+ * would queue thousands); every other taxel is 0 (the output is cleared
+ * before). */
+#define HCS_TXB_LIVE 1024
+__global__ __launch_bounds__(HCS_TXB_LIVE) void k_txb_live(const unsigned* __restrict__ cnt,
+                                                           const unsigned* __restrict__ off, int ncell,
+                                                           int64_t list_cap, int* __restrict__ live,
+                                                           unsigned* __restrict__ nlive) {
+  __shared__ unsigned s_off[HCS_TXB_LIVE / 64 + 1];
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = cell < ncell && cnt[cell] > 0 && (int64_t)off[cell] + cnt[cell] <= list_cap;
   const unsigned long long m = __ballot(on);
-  const int lane = threadIdx.x & 63;
-  unsigned base = 0;
-  if (lane == 0 && m) base = atomicAdd(nlive, (unsigned)__popcll(m));
-  base = __shfl(base, 0, 64);
-  if (on) live[base + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = cell;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) s_off[w] = (unsigned)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0;
+    for (int x = 0; x < (int)(blockDim.x >> 6); ++x) {
+      const unsigned n = s_off[x];
+      s_off[x] = tot;
+      tot += n;
+    }
+    s_off[HCS_TXB_LIVE / 64] = tot ? atomicAdd(nlive, tot) : 0u;
+  }
+  __syncthreads();
+  if (on)
+    live[s_off[HCS_TXB_LIVE / 64] + s_off[w] +
+         __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = cell;
 }
 
 /* One wave per live taxel of any sensor, strided over the grid; FAN: the
@@ -324,7 +340,8 @@ void batch_taxels_enqueue(Ctx* c, BatchTaxels& t, const double* state) {
   unsigned* nlive = (unsigned*)t.poly_cnt.ptr + 2;
   launch_fill_u32(s, nlive, 0u, 1);
   launch_fill_u32(s, (unsigned*)t.out.ptr, 0u, 2 * t.ncell);
-  hipLaunchKernelGGL(k_txb_live, dim3((t.ncell + 255) / 256), dim3(256), 0, s, (const unsigned*)t.cnt.ptr,
+  hipLaunchKernelGGL(k_txb_live, dim3((t.ncell + HCS_TXB_LIVE - 1) / HCS_TXB_LIVE), dim3(HCS_TXB_LIVE), 0, s,
+                     (const unsigned*)t.cnt.ptr,
                      (const unsigned*)t.off.ptr, t.ncell, t.list_cap, (int*)t.live.ptr, nlive);
   const dim3 grid(std::max(1, std::min(t.ncell, c->num_cu * 16)));
   if (t.max_rays <= HCS_TX_RAYS)
