@@ -290,6 +290,12 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
  * ray-major loop's arithmetic on the same values, so the chosen triangle,
  * and the taxel, are the same. */
 #define HCS_TX_RAYS 576
+/* a lane takes HCS_TX_ROWS rows of a fan's range at a time; the rows past
+ * a fan's first piece are queued for the whole wave, at most 5 pieces per
+ * fan of a round of 64 (S <= 24 rows) */
+#define HCS_TX_ROWS 4
+#define HCS_TX_TASKS (64 * ((24 + HCS_TX_ROWS - 1) / HCS_TX_ROWS - 1))
+static_assert(HCS_TX_RAYS == 24 * 24, "rays of the fan-major sampler: at most 24 rows of 24");
 struct TxScatter {
   unsigned long long tmin[HCS_TX_RAYS], smin[HCS_TX_RAYS];
   /* first pass: list position * 8 + fan of the hit that lowered t last;
@@ -298,6 +304,9 @@ struct TxScatter {
   unsigned char flag[HCS_TX_RAYS]; /* the ray needs the second pass */
   int q[HCS_TX_CHUNK], fan0[HCS_TX_CHUNK + 1];
   long long kq[HCS_TX_CHUNK];
+  /* pieces of HCS_TX_ROWS rows of the round's fans past their first
+   * piece: fan * 32 + first row */
+  int tq[HCS_TX_TASKS];
 };
 
 /* Hit-test constants of fan k of polygon P (the ray-major loop's e1, e2,
@@ -437,46 +446,93 @@ __device__ inline double fan_hit_at(const double* P, int nv, int k, const TaxelG
   return fan_hit(c, g, x, y, i, j);
 }
 
-/* One pass over the fans of list[base + c0 ..] for the wave: PASS 0 keeps
- * the least t per ray, PASS 1 the least (key, fan) among hits at that t. */
+/* The entry of fan f of the staged chunk: the last entry whose first fan
+ * is at most f (binary search; every entry has at least one fan). */
+__device__ inline int fan_entry(const TxScatter& L, unsigned cn, int f) {
+  int l = 0;
+  for (int hi = (int)cn - 1; l < hi;) {
+    const int mid = (l + hi + 1) >> 1;
+    if (L.fan0[mid] <= f) l = mid;
+    else hi = mid - 1;
+  }
+  return l;
+}
+
+/* Rows ia..ib of fan c (entry l, fan k of the chunk starting at list
+ * position c0): PASS 0 keeps the least t per ray, PASS 1 the least (key,
+ * fan) among hits at that t. */
+template <int PASS>
+__device__ inline void fan_rows(TxScatter& L, const TxFan& c, const TaxelGridDev& g, int x, int y, int ia, int ib,
+                                unsigned c0, int l, int k) {
+  const unsigned long long sk = ((unsigned long long)L.kq[l] << 3) | (unsigned)k;
+  for (int i = ia; i <= ib; ++i) {
+    int ja, jb;
+    fan_row(c, g, x, y, i, ja, jb);
+    for (int j = ja; j <= jb; ++j) {
+      const int s = i * g.S + j;
+      if (PASS == 1 && !L.flag[s]) continue;
+      const double t = fan_hit(c, g, x, y, i, j);
+      if (!(t > 0.0)) continue;
+      const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
+      if (PASS == 0) {
+        const unsigned long long old = atomicMin(&L.tmin[s], tb);
+        if (tb < old) L.own[s] = (int)(((c0 + (unsigned)l) << 3) | (unsigned)k);
+        else if (tb == old) L.flag[s] = 1;
+      } else if (tb == L.tmin[s]) {
+        atomicMin(&L.smin[s], sk);
+      }
+    }
+  }
+}
+
+/* One pass over the fans of list[base + c0 ..] for the wave, 64 fans a
+ * round: each lane takes the first HCS_TX_ROWS rows of its fan's range and
+ * queues the rest in pieces of HCS_TX_ROWS rows, which the wave then
+ * shares, so a large fan does not hold the round while the other lanes
+ * idle.  Pieces cover disjoint rows of one fan, and the per-ray minima do
+ * not depend on the order of the hits, so the result is that of one lane
+ * per fan. */
 template <int PASS>
 __device__ inline void scatter_chunk(TxScatter& L, const double* __restrict__ poly, const TaxelGridDev& g, int x,
                                      int y, unsigned cn, unsigned c0) {
   const int lane = threadIdx.x & 63;
   const int nf = L.fan0[cn];
-  for (int f = lane; f < nf; f += 64) {
-    /* the entry of fan f: the last entry whose first fan is at most f
-     * (binary search; every entry has at least one fan) */
-    int l = 0;
-    for (int hi = (int)cn - 1; l < hi;) {
-      const int mid = (l + hi + 1) >> 1;
-      if (L.fan0[mid] <= f) l = mid;
-      else hi = mid - 1;
-    }
-    const int k = f - L.fan0[l];
-    const double* P = poly + (size_t)L.q[l] * HCS_PREC;
-    const int nv = min((int)P[0], 8);
-    TxFan c;
-    if (!fan_setup(P, nv, k, g, x, y, c)) continue;
-    const unsigned long long sk = ((unsigned long long)L.kq[l] << 3) | (unsigned)k;
-    for (int i = c.i0; i <= c.i1; ++i) {
-      int ja, jb;
-      fan_row(c, g, x, y, i, ja, jb);
-      for (int j = ja; j <= jb; ++j) {
-        const int s = i * g.S + j;
-        if (PASS == 1 && !L.flag[s]) continue;
-        const double t = fan_hit(c, g, x, y, i, j);
-        if (!(t > 0.0)) continue;
-        const unsigned long long tb = (unsigned long long)__double_as_longlong(t);
-        if (PASS == 0) {
-          const unsigned long long old = atomicMin(&L.tmin[s], tb);
-          if (tb < old) L.own[s] = (int)(((c0 + (unsigned)l) << 3) | (unsigned)k);
-          else if (tb == old) L.flag[s] = 1;
-        } else if (tb == L.tmin[s]) {
-          atomicMin(&L.smin[s], sk);
-        }
+  for (int f0 = 0; f0 < nf; f0 += 64) {
+    const int f = f0 + lane;
+    unsigned extra = 0;
+    int i0 = 0;
+    if (f < nf) {
+      const int l = fan_entry(L, cn, f);
+      const int k = f - L.fan0[l];
+      const double* P = poly + (size_t)L.q[l] * HCS_PREC;
+      TxFan c;
+      if (fan_setup(P, min((int)P[0], 8), k, g, x, y, c)) {
+        i0 = c.i0;
+        fan_rows<PASS>(L, c, g, x, y, c.i0, min(c.i1, c.i0 + HCS_TX_ROWS - 1), c0, l, k);
+        extra = (unsigned)((c.i1 - c.i0) / HCS_TX_ROWS);
       }
     }
+    /* queue the further pieces by a prefix sum over the wave */
+    unsigned incl = extra;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned t = __shfl(incl, lane >= o ? lane - o : lane, 64);
+      if (lane >= o) incl += t;
+    }
+    const int tot = (int)__shfl(incl, 63, 64);
+    if (tot == 0) continue;
+    for (unsigned m = 0; m < extra; ++m) L.tq[incl - extra + m] = (f << 5) | (i0 + (int)(m + 1) * HCS_TX_ROWS);
+    __syncthreads();
+    for (int t = lane; t < tot; t += 64) {
+      const int ff = L.tq[t] >> 5, r = L.tq[t] & 31;
+      const int l = fan_entry(L, cn, ff);
+      const int k = ff - L.fan0[l];
+      const double* P = poly + (size_t)L.q[l] * HCS_PREC;
+      TxFan c;
+      if (fan_setup(P, min((int)P[0], 8), k, g, x, y, c))
+        fan_rows<PASS>(L, c, g, x, y, r, min(c.i1, r + HCS_TX_ROWS - 1), c0, l, k);
+    }
+    __syncthreads();
   }
 }
 
