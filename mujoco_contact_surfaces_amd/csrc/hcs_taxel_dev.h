@@ -291,9 +291,9 @@ __device__ inline double sample_cell(const double* __restrict__ poly, const doub
  * and the taxel, are the same. */
 #define HCS_TX_RAYS 576
 /* a lane takes HCS_TX_ROWS rows of a fan's range at a time; the rows past
- * a fan's first piece are queued for the whole wave, at most 5 pieces per
+ * a fan's first piece are queued for the whole wave, at most 11 pieces per
  * fan of a round of 64 (S <= 24 rows) */
-#define HCS_TX_ROWS 4
+#define HCS_TX_ROWS 2
 #define HCS_TX_TASKS (64 * ((24 + HCS_TX_ROWS - 1) / HCS_TX_ROWS - 1))
 static_assert(HCS_TX_RAYS == 24 * 24, "rays of the fan-major sampler: at most 24 rows of 24");
 struct TxScatter {
